@@ -1,0 +1,214 @@
+/*
+ * sgw.h — C ABI of libsgw_hip.so, the MI355X (gfx950) implementation of the
+ * Sternheimer linear-response solve of SternheimerGW.
+ *
+ * Plain C: pointers and sizes only.  Complex numbers are interleaved
+ * (re, im) float64 pairs, layout-compatible with Fortran COMPLEX(dp),
+ * C `double _Complex` and hipDoubleComplex.  Arrays are column-major in the
+ * Fortran sense: a solution block xx(n, nshift) has xx(:, k) at xx + k*n.
+ *
+ * Return values: a non-negative value is the solver's error code `ierr`
+ * with the reference's meaning (0 converged, 1 not converged within
+ * max_iter, 2 NaN in the solution); a negative value is an API error
+ * (SGW_E_*) and sgw_last_error() describes it.  Where the reference aborts
+ * through `errore` (shape violations), this library returns SGW_E_SHAPE.
+ *
+ * Every entry point runs its arithmetic on the GPU; none falls back to the
+ * CPU.  The library is not thread-safe per operator/solver object.
+ */
+#ifndef SGW_H
+#define SGW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } sgw_complex;
+
+#define SGW_OK 0
+#define SGW_E_ARG (-1)
+#define SGW_E_SHAPE (-2)
+#define SGW_E_HIP (-3)
+#define SGW_E_ALLOC (-4)
+#define SGW_E_STATE (-5)
+
+/* Solver identifiers of select_solver (algo/linear_solver/src/select_solver.f90:39-45). */
+#define SGW_BICGSTAB_MULTI 1
+#define SGW_BICGSTAB_NO_MULTI 2
+#define SGW_LINEAR_SOLVER 3
+
+/* The operator procedure argument AA(sigma, xx, Ax): Ax = (A + sigma) xx.
+ * Replaces the Fortran interface block at
+ * algo/linear_solver/src/select_solver.f90:77-87 (same at bicgstab.f90:164-174,
+ * linear_solver.f90:91-101).  x and Ax are host arrays of length n. */
+typedef void (*sgw_op_fn)(const sgw_complex *sigma, const sgw_complex *x, sgw_complex *Ax,
+                          int64_t n, void *ctx);
+
+/* TYPE(select_solver_type), algo/linear_solver/src/select_solver.f90:48-62.
+ * Defaults of the reference: max_iter 10000, threshold 1e-4, bicg_lmax 4;
+ * `priority` must be given (the reference aborts when it is unallocated). */
+typedef struct {
+  const int32_t *priority;
+  int32_t npriority;
+  int32_t max_iter;
+  double threshold;
+  int32_t bicg_lmax;
+} sgw_select_solver_config;
+
+/* TYPE(bicgstab_type), algo/linear_solver/src/bicgstab.f90:44-55. */
+typedef struct {
+  int32_t lmax;
+  double threshold;
+  int32_t max_iter;
+} sgw_bicgstab_config;
+
+/* TYPE(linear_solver_config), algo/linear_solver/src/linear_solver.f90:42-56.
+ * abs_threshold is set on output (the reference's INTENT(INOUT) config). */
+typedef struct {
+  int32_t recover_subspace;
+  int32_t max_iter;
+  double threshold;
+  double abs_threshold;
+} sgw_linear_solver_config;
+
+/* ---------------------------------------------------------------------------
+ * Drop-in solver entry points with a host operator callback.
+ * bb[n], sigma[nshift], xx[n*nshift] are host arrays.
+ * ------------------------------------------------------------------------- */
+
+/* select_solver(config, AA, bb, sigma, xx, ierr),
+ * algo/linear_solver/src/select_solver.f90:67-161. */
+int sgw_select_solver(const sgw_select_solver_config *config, sgw_op_fn AA, void *ctx,
+                      const sgw_complex *bb, int64_t n, const sgw_complex *sigma,
+                      int32_t nshift, sgw_complex *xx);
+
+/* bicgstab(config, AA, bb, sigma, xx, ierr), algo/linear_solver/src/bicgstab.f90:154-275.
+ * iters (nullable) receives the outer iteration count at exit. */
+int sgw_bicgstab(const sgw_bicgstab_config *config, sgw_op_fn AA, void *ctx,
+                 const sgw_complex *bb, int64_t n, const sgw_complex *sigma, int32_t nshift,
+                 sgw_complex *xx, int32_t *iters);
+
+/* linear_solver(config, AA, bb, sigma, xx, ierr),
+ * algo/linear_solver/src/linear_solver.f90:82-194. */
+int sgw_linear_solver(sgw_linear_solver_config *config, sgw_op_fn AA, void *ctx,
+                      const sgw_complex *bb, int64_t n, const sgw_complex *sigma,
+                      int32_t nshift, sgw_complex *xx);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident operators.  Replaces the host callback by an operator that
+ * lives on the GPU, so the solvers never leave the device.
+ * ------------------------------------------------------------------------- */
+typedef struct sgw_operator sgw_operator;
+
+/* Dense operator A (n x n, column-major, host array copied to the device):
+ * Ax = MATMUL(A, x) + sigma x — the reference's unit-test operator
+ * linear_problem_apply, algo/linear_solver/test/linear_solver.pf:100-113. */
+sgw_operator *sgw_operator_dense(const sgw_complex *a, int64_t n);
+
+/* Host callback wrapped as an operator (vectors cross PCIe on every call). */
+sgw_operator *sgw_operator_callback(sgw_op_fn AA, void *ctx, int64_t n);
+
+/* Plane-wave shifted Hamiltonian of linear_op (algo/linear_solver/src/linear_op.f90:46-206)
+ * with QE's h_psi (kinetic + local via FFT + nonlocal projectors), in Rydberg:
+ *   Ax = |k+q+G|^2 x + fwfft(V_loc(r) invfft(x)) + sum_p proj_p D_p <proj_p|x> + sigma x.
+ * The projector list of group g holds the beta functions (D = the diagonal
+ * nonlocal coefficients) followed, for the Coulomb operator, by the valence
+ * states at k+q with D = alpha_pv, which is the alpha_pv P_v term of
+ * linear_op.f90:131-138 (S = 1, norm-conserving).  One group per k+q point. */
+typedef struct {
+  int32_t n1, n2, n3;          /* FFT box                                          */
+  const double *vloc;          /* V_loc(r), n1*n2*n3, index i1 + n1*(i2 + n2*i3)   */
+  int32_t ngroup;              /* number of k+q points                             */
+  const int32_t *npw;          /* plane waves per group                            */
+  int64_t ld;                  /* padded vector length, >= max npw                 */
+  const int32_t *mill;         /* [ngroup][ld][3] Miller indices (m1, m2, m3)       */
+  const double *kin;           /* [ngroup][ld] |k+q+G|^2 in Ry                     */
+  int32_t nproj;               /* projectors per group                             */
+  const sgw_complex *proj;     /* [ngroup][nproj][ld]                              */
+  const double *dcoef;         /* [ngroup][nproj] diagonal coefficients (Ry)       */
+} sgw_pw_desc;
+sgw_operator *sgw_operator_pw(const sgw_pw_desc *desc);
+
+void sgw_operator_destroy(sgw_operator *op);
+
+/* ax[v] = (A_{group[v]} + sigma[v]) x[v], v < nvec; host arrays with leading
+ * dimension ldx (>= operator ld).  group may be NULL for one-group operators. */
+int sgw_operator_apply(sgw_operator *op, int32_t nvec, const int32_t *group,
+                       const sgw_complex *sigma, const sgw_complex *x, int64_t ldx,
+                       sgw_complex *ax);
+
+/* Same as sgw_operator_apply on device arrays, on `stream` (hipStream_t, NULL
+ * = default stream); no synchronisation. */
+int sgw_operator_apply_device(sgw_operator *op, int32_t nvec, const int32_t *d_group,
+                              const sgw_complex *d_sigma, const sgw_complex *d_x, int64_t ldx,
+                              sgw_complex *d_ax, void *stream);
+
+/* select_solver on a device operator (group selects the k+q point). */
+int sgw_select_solver_op(const sgw_select_solver_config *config, sgw_operator *op,
+                         int32_t group, const sgw_complex *bb, int64_t n,
+                         const sgw_complex *sigma, int32_t nshift, sgw_complex *xx);
+
+/* ---------------------------------------------------------------------------
+ * Batched multishift Sternheimer solve: nsys independent systems
+ *   (A_{group[s]} + sigma[s][k]) x[s][k] = rhs[s],  k < nshift,
+ * each advanced by the multishift BiCGStab(l) recurrences of bicgstab.f90
+ * with its own convergence test (||r_seed|| < threshold, bicgstab.f90:237,245),
+ * all systems in every kernel launch.  This is the batched form of the band
+ * loop of solve_linter (phys/coul/src/solve_linter.f90:367-374, sigma =
+ * -(eps_v + omega)) and of the G' loop of green_function
+ * (phys/green/src/green.f90:196-218, sigma = -omega).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+  int32_t nsys;        /* systems (bands x q, or G' columns)           */
+  int32_t nshift;      /* shifts per system; shift 0 is the seed      */
+  int64_t ld;          /* leading dimension of rhs and x (>= op ld)   */
+  int32_t lmax;        /* BiCGStab(l) degree, 1..15                   */
+  int32_t max_iter;
+  double threshold;    /* absolute, on the seed residual              */
+} sgw_batch_config;
+
+typedef struct sgw_batch_solver sgw_batch_solver;
+
+sgw_batch_solver *sgw_batch_solver_create(sgw_operator *op, const sgw_batch_config *cfg);
+void sgw_batch_solver_destroy(sgw_batch_solver *solver);
+
+/* Device arrays: d_group[nsys], d_rhs[nsys][ld], d_sigma[nsys][nshift],
+ * d_x[nsys][nshift][ld]; h_sigma is a host copy of d_sigma.  h_ierr and
+ * h_iters (host, nullable) receive per-system ierr and iteration counts.
+ * Synchronises `stream` before returning. */
+int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
+                         const sgw_complex *d_rhs, const sgw_complex *d_sigma,
+                         const sgw_complex *h_sigma, sgw_complex *d_x, int32_t *h_ierr,
+                         int32_t *h_iters, void *stream);
+
+/* One-shot form on host arrays (copies in and out). */
+int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
+                                  const int32_t *group, const sgw_complex *rhs,
+                                  const sgw_complex *sigma, sgw_complex *x, int32_t *ierr,
+                                  int32_t *iters);
+
+/* True relative residuals ||rhs[s] - (A + sigma[s][k]) x[s][k]|| / ||rhs[s]||
+ * recomputed in FP64 on the device for every (system, shift); device arrays,
+ * result d_res[nsys][nshift]. */
+int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, int64_t ld,
+                              const int32_t *d_group, const sgw_complex *d_rhs,
+                              const sgw_complex *d_sigma, const sgw_complex *d_x,
+                              double *d_res, void *stream);
+
+/* Kernel timing with HIP events recorded on each kernel's launch stream.
+ * sgw_kernel_timing(1) enables and resets, (0) disables and resets; query
+ * returns the summed duration (ms) and launch count of one kernel name
+ * ("pw_plane", "pw_assemble", "bicg_v1", "bicg_v2", "mr_mgs", "mr_v3",
+ * "dotu", "dense_matvec"). */
+int sgw_kernel_timing(int enable);
+int sgw_kernel_timing_query(const char *name, double *total_ms, int64_t *launches);
+
+const char *sgw_last_error(void);
+const char *sgw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,675 @@
+// Batched multishift BiCGStab(l) on the GPU.
+//
+// Algorithm: Frommer, Computing 70, 87 (2003), as implemented by SternheimerGW
+// algo/linear_solver/src/bicgstab.f90:154-999 (Algorithms 2-5).  theta, phi
+// and xi are carried as inverses (bicgstab.f90:30-32).  The seed system is the
+// first shift; every other shift is carried as sigma_k - sigma_1 (:458).
+//
+// Batching: S independent systems (band x q) advance in lock step.  Each
+// system freezes as soon as the reference's convergence test
+// (||r_seed|| < threshold, :237,:245) passes, so every system performs exactly
+// the iterations the per-system reference would.  All vector state lives in
+// HBM; the scalar recurrences (rho, alpha, omega, phi, theta, mu, gamma...)
+// run on the device in FP64, one thread per system.
+//
+// Layout (ld = padded vector length, zero padding is preserved by all
+// updates):  R,U: [S][l+1][ld]   X,RT: [S][ld]
+//            SU: [S][nsh][l+1][ld]   SX: [S][nsh][ld]
+#include "sgw_internal.h"
+#include "bicgstab.h"
+#include "ktimer.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace sgw {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ inline double2 wave_sum(double2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
+// Block-wide sum (deterministic); every thread returns the total.
+template <int NT> __device__ inline double2 block_sum(double2 v, double2 *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  double2 t = mk(0, 0);
+  if (threadIdx.x < 64) {
+    t = threadIdx.x < NT / 64 ? lds[threadIdx.x] : mk(0, 0);
+    t = wave_sum(t);
+    if (threadIdx.x == 0) lds[0] = t;
+  }
+  __syncthreads();
+  t = lds[0];
+  return t;
+}
+
+// out[s] = sum_i a[s][i] * b[s][i]   (ZDOTU) over ld entries
+__global__ __launch_bounds__(kBlock) void k_dotu(int nsys, const int32_t *__restrict__ active,
+                                                 const double2 *__restrict__ a, int64_t sa,
+                                                 const double2 *__restrict__ b, int64_t sb,
+                                                 int64_t n, double2 *__restrict__ out) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 *pa = a + s * sa, *pb = b + s * sb;
+  double2 acc = mk(0, 0);
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) cmac(acc, pa[i], pb[i]);
+  acc = block_sum<kBlock>(acc, lds);
+  if (threadIdx.x == 0) out[s] = acc;
+}
+
+// out[s] = sqrt(sum_i |a[s][i]|^2)
+__global__ __launch_bounds__(kBlock) void k_norm(int nsys, const int32_t *__restrict__ active,
+                                                 const double2 *__restrict__ a, int64_t sa,
+                                                 int64_t n, double *__restrict__ out) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 *pa = a + s * sa;
+  double2 acc = mk(0, 0);
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) {
+    double2 v = pa[i];
+    acc.x = fma(v.x, v.x, acc.x);
+    acc.x = fma(v.y, v.y, acc.x);
+  }
+  acc = block_sum<kBlock>(acc, lds);
+  if (threadIdx.x == 0) out[s] = sqrt(acc.x);
+}
+
+// ---------------------------------------------------------------------------
+// scalar recurrences (one thread per system)
+// ---------------------------------------------------------------------------
+struct Scal {
+  // seed, per system
+  double2 *rho, *rho_old, *alpha, *alpha_old, *beta, *omega, *dot;
+  double2 *gamma, *gamma_p, *gamma_pp;  // [S][l]
+  // shifts, per (system, shift): index s*nsh + k
+  double2 *sigma, *inv_phi_old, *inv_phi, *inv_phi_new, *inv_theta, *alpha_s, *beta_s;
+  double2 *fa, *fb, *ia;       // factors for L15/L19, L26 and 1/alpha^sigma
+  double2 *mu;                 // [S*nsh][l(l+1)/2]
+  double2 *coef;               // [S*nsh][l+2]: gamma'_1/xi, gamma''_j/xi (j=1..l-1), 1/psi
+};
+
+// bicgstab.f90:587  rho_old = -omega * rho_old
+__global__ void k_bicg_begin(int nsys, const int32_t *__restrict__ active, Scal sc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || !active[s]) return;
+  sc.rho_old[s] = cmul(cneg(sc.omega[s]), sc.rho_old[s]);
+}
+
+// bicgstab.f90:595-599  rho = (r_j, r0~); beta = alpha rho / rho_old; rho_old = rho
+__global__ void k_bicg_s1(int nsys, const int32_t *__restrict__ active, Scal sc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || !active[s]) return;
+  double2 rho = sc.dot[s];
+  sc.rho[s] = rho;
+  sc.beta[s] = cdiv(cmul(sc.alpha[s], rho), sc.rho_old[s]);
+  sc.rho_old[s] = rho;
+}
+
+// bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars
+__global__ void k_bicg_s2(int nsys, int nsh, const int32_t *__restrict__ active, Scal sc) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || !active[s]) return;
+  const double2 one = mk(1, 0);
+  double2 alpha = cdiv(sc.rho[s], sc.dot[s]);
+  sc.alpha[s] = alpha;
+  double2 beta = sc.beta[s], alpha_old = sc.alpha_old[s];
+  double2 ab = cdiv(cmul(alpha, beta), alpha_old);
+  for (int k = 0; k < nsh; ++k) {
+    int i = s * nsh + k;
+    double2 ipo = sc.inv_phi_old[i], ip = sc.inv_phi[i];
+    double2 t = csub(cdiv(ip, ipo), one);
+    double2 den = cadd(cadd(one, cmul(alpha, sc.sigma[i])), cmul(ab, t));
+    double2 ipn = cdiv(ip, den);                                   // L13
+    double2 q = cdiv(ip, ipo);
+    double2 beta_s = cmul(cmul(q, q), beta);
+    double2 alpha_s = cmul(cdiv(ipn, ip), alpha);
+    double2 th = sc.inv_theta[i];
+    sc.fa[i] = cmul(th, ip);                                       // 1/(theta phi) before L18
+    sc.beta_s[i] = beta_s;
+    sc.alpha_s[i] = alpha_s;
+    sc.inv_phi_new[i] = ipn;
+    sc.inv_phi_old[i] = ip;                                        // L18
+    sc.inv_phi[i] = ipn;
+    sc.fb[i] = cneg(cmul(th, ipn));                                // L26 factor
+    sc.ia[i] = cdiv(one, alpha_s);
+  }
+  sc.alpha_old[s] = alpha;                                         // L18 (seed)
+}
+
+// ---------------------------------------------------------------------------
+// element-wise vector kernels; grid = (ceil(ld/kBlock), S)
+// ---------------------------------------------------------------------------
+struct Vecs {
+  double2 *R, *U, *X, *RT, *SU, *SX;
+  int64_t ld;
+  int l, nsh;
+  __device__ double2 *r(int s, int i) const { return R + ((int64_t)s * (l + 1) + i) * ld; }
+  __device__ double2 *u(int s, int i) const { return U + ((int64_t)s * (l + 1) + i) * ld; }
+  __device__ double2 *x(int s) const { return X + (int64_t)s * ld; }
+  __device__ double2 *su(int s, int k, int i) const {
+    return SU + (((int64_t)s * nsh + k) * (l + 1) + i) * ld;
+  }
+  __device__ double2 *sx(int s, int k) const { return SX + ((int64_t)s * nsh + k) * ld; }
+};
+
+// L7-L8: u_i = r_i - beta u_i, i = 0..j
+__global__ __launch_bounds__(kBlock) void k_bicg_v1(int j, const int32_t *__restrict__ active,
+                                                    Vecs v, Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  const double2 mb = cneg(sc.beta[s]);
+  for (int i = 0; i <= j; ++i) {
+    double2 *pu = v.u(s, i) + e;
+    double2 t = cmul(mb, *pu);
+    *pu = cadd(t, v.r(s, i)[e]);
+  }
+}
+
+// L14-L19 (shifts), L21-L22 (seed residuals), L25 (seed x), L26 (shifts).
+__global__ __launch_bounds__(kBlock) void k_bicg_v2(int j, const int32_t *__restrict__ active,
+                                                    Vecs v, Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  constexpr int LM = kMaxL + 1;
+  double2 rr[LM];
+  for (int i = 0; i <= j; ++i) rr[i] = v.r(s, i)[e];
+  // seed residual update r_i -= alpha u_{i+1} (uses the updated u's)
+  const double2 alpha = sc.alpha[s];
+  const double2 malpha = cneg(alpha);
+  double2 rnew[LM];
+  for (int i = 0; i <= j; ++i) {
+    double2 t = rr[i];
+    cmac(t, malpha, v.u(s, i + 1)[e]);
+    rnew[i] = t;
+    v.r(s, i)[e] = t;
+  }
+  // L25 x += alpha u_0
+  {
+    double2 *px = v.x(s) + e;
+    double2 t = *px;
+    cmac(t, alpha, v.u(s, 0)[e]);
+    *px = t;
+  }
+  for (int k = 0; k < v.nsh; ++k) {
+    const int idx = s * v.nsh + k;
+    const double2 mbs = cneg(sc.beta_s[idx]);
+    const double2 fa = sc.fa[idx];
+    double2 u0 = mk(0, 0);
+    for (int i = 0; i <= j; ++i) {                       // L15
+      double2 *pu = v.su(s, k, i) + e;
+      double2 t = cmul(mbs, *pu);
+      cmac(t, fa, rr[i]);
+      *pu = t;
+      if (i == 0) u0 = t;
+    }
+    {                                                    // L17
+      double2 *px = v.sx(s, k) + e;
+      double2 t = *px;
+      cmac(t, sc.alpha_s[idx], u0);
+      *px = t;
+    }
+    // L19 then L26: u_{j+1} = ((fa r_j) + fb r_j_new) / alpha^s - sigma u_j
+    double2 uj1 = cmul(fa, rr[j]);
+    cmac(uj1, sc.fb[idx], rnew[j]);
+    uj1 = cmul(sc.ia[idx], uj1);
+    cmac(uj1, cneg(sc.sigma[idx]), v.su(s, k, j)[e]);
+    v.su(s, k, j + 1)[e] = uj1;
+  }
+}
+
+// Algorithm 4, MGS part and all scalars (one block per system).
+// bicgstab.f90:780-829 (seed) and :850-884 (per-shift Horner and gammas).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mr_mgs(const int32_t *__restrict__ active, Vecs v, Scal sc,
+                                               int ntau, int nmu) {
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  __shared__ double2 lds[NT / 64];
+  __shared__ double2 tau[(kMaxL + 1) * (kMaxL + 2) / 2];
+  __shared__ double2 nu[kMaxL + 1];
+  __shared__ double2 gam[kMaxL + 1], gp[kMaxL + 1], gpp[kMaxL + 1];
+  const int l = v.l;
+  const int64_t ld = v.ld;
+  for (int jj = 1; jj <= l; ++jj) {
+    double2 *rj = v.r(s, jj);
+    int offset = jj * (jj + 1) / 2 + 1;
+    for (int ii = 1; ii <= jj - 1; ++ii) {
+      const double2 *ri = v.r(s, ii);
+      double2 acc = mk(0, 0);
+      for (int64_t e = threadIdx.x; e < ld; e += NT) cmac(acc, rj[e], ri[e]);
+      acc = block_sum<NT>(acc, lds);
+      double2 t = cdiv(acc, nu[ii]);
+      if (threadIdx.x == 0) tau[offset + ii - 1] = t;
+      const double2 mt = cneg(t);
+      for (int64_t e = threadIdx.x; e < ld; e += NT) {
+        double2 x = rj[e];
+        cmac(x, mt, ri[e]);
+        rj[e] = x;
+      }
+      __syncthreads();
+    }
+    double2 a1 = mk(0, 0), a2 = mk(0, 0);
+    const double2 *r0 = v.r(s, 0);
+    for (int64_t e = threadIdx.x; e < ld; e += NT) {
+      double2 x = rj[e];
+      cmac(a1, x, x);
+      cmac(a2, r0[e], x);
+    }
+    a1 = block_sum<NT>(a1, lds);
+    a2 = block_sum<NT>(a2, lds);
+    if (threadIdx.x == 0) {
+      nu[jj] = a1;
+      gp[jj] = cdiv(a2, a1);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    gam[l] = gp[l];                                                // L7
+    sc.omega[s] = gam[l];
+    for (int jj = l - 1; jj >= 1; --jj) {                          // L8-9
+      double2 g = gp[jj];
+      for (int ii = jj + 1; ii <= l; ++ii) {
+        int ij = ii * (ii + 1) / 2 + jj + 1;
+        g = csub(g, cmul(tau[ij - 1], gam[ii]));
+      }
+      gam[jj] = g;
+    }
+    for (int jj = 1; jj <= l - 1; ++jj) {                          // L11-12
+      double2 g = gam[jj + 1];
+      for (int ii = jj + 1; ii <= l - 1; ++ii) {
+        int ij = ii * (ii + 1) / 2 + jj + 1;
+        g = cadd(g, cmul(tau[ij - 1], gam[ii + 1]));
+      }
+      gpp[jj] = g;
+    }
+    for (int jj = 1; jj <= l; ++jj) {
+      sc.gamma[s * l + jj - 1] = gam[jj];
+      sc.gamma_p[s * l + jj - 1] = gp[jj];
+      sc.gamma_pp[s * l + jj - 1] = (jj < l) ? gpp[jj] : mk(0, 0);
+    }
+  }
+  __syncthreads();
+  // per shift (one thread each): Horner (Algorithm 5), theta, gamma', gamma''
+  for (int k = threadIdx.x; k < v.nsh; k += NT) {
+    const int idx = s * v.nsh + k;
+    const double2 sg = sc.sigma[idx], one = mk(1, 0);
+    double2 g[kMaxL + 1];
+    g[l] = cneg(gam[l]);
+    for (int jj = l - 1; jj >= 1; --jj) g[jj] = csub(cmul(cneg(sg), g[jj + 1]), gam[jj]);
+    double2 psi = cadd(cmul(cneg(sg), g[1]), one);
+    for (int ii = 1; ii <= l - 1; ++ii)
+      for (int jj = l - 1; jj >= ii; --jj) g[jj] = cadd(cmul(cneg(sg), g[jj + 1]), g[jj]);
+    for (int jj = 1; jj <= l; ++jj) g[jj] = cdiv(cneg(g[jj]), psi);
+    double2 inv_xi = cmul(sc.inv_theta[idx], sc.inv_phi[idx]);   // L21
+    sc.inv_theta[idx] = cdiv(sc.inv_theta[idx], psi);
+    const double2 *mu = sc.mu + (int64_t)idx * nmu;
+    double2 gps[kMaxL + 1];
+    for (int jj = 1; jj <= l; ++jj) {                              // L22-23
+      double2 a = mk(0, 0);
+      for (int ii = jj; ii <= l; ++ii) {
+        int ij = (ii - 1) * ii / 2 + jj;
+        a = cadd(a, cmul(mu[ij - 1], g[ii]));
+      }
+      gps[jj] = a;
+    }
+    double2 *cf = sc.coef + (int64_t)idx * (kMaxL + 2);
+    cf[0] = cmul(gps[1], inv_xi);                                  // L28 factor
+    for (int jj = 1; jj <= l - 1; ++jj) {                          // L25-26, L31 factor
+      double2 a = gps[jj + 1];
+      for (int ii = jj + 1; ii <= l - 1; ++ii) {
+        int ij = ii * (ii + 1) / 2 + jj + 1;
+        a = cadd(a, cmul(tau[ij - 1], gps[ii + 1]));
+      }
+      cf[jj] = cmul(a, inv_xi);
+    }
+    cf[l] = cdiv(one, psi);                                        // L34 factor
+  }
+  (void)ntau;
+}
+
+// Algorithm 4 vector updates: seed x, u_0 (L14-17), shifted x, u_0 (L28-34),
+// delayed r_0 update (L32).  bicgstab.f90:833-925.
+__global__ __launch_bounds__(kBlock) void k_mr_v3(const int32_t *__restrict__ active, Vecs v,
+                                                  Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  const int l = v.l;
+  double2 rr[kMaxL + 1];
+  for (int i = 0; i <= l; ++i) rr[i] = v.r(s, i)[e];
+  const double2 *gam = sc.gamma + s * l, *gp = sc.gamma_p + s * l, *gpp = sc.gamma_pp + s * l;
+  {
+    double2 x = v.x(s)[e];
+    cmac(x, gam[0], rr[0]);                                        // L14
+    double2 u0 = v.u(s, 0)[e];
+    cmac(u0, cneg(gam[l - 1]), v.u(s, l)[e]);
+    for (int jj = 1; jj <= l - 1; ++jj) {                          // L15-17
+      cmac(u0, cneg(gam[jj - 1]), v.u(s, jj)[e]);
+      cmac(x, gpp[jj - 1], rr[jj]);
+    }
+    v.x(s)[e] = x;
+    v.u(s, 0)[e] = u0;
+  }
+  for (int k = 0; k < v.nsh; ++k) {
+    const int idx = s * v.nsh + k;
+    const double2 *cf = sc.coef + (int64_t)idx * (kMaxL + 2);
+    double2 x = v.sx(s, k)[e];
+    cmac(x, cf[0], rr[0]);                                         // L28
+    double2 u0 = v.su(s, k, 0)[e];
+    cmac(u0, cneg(gam[l - 1]), v.su(s, k, l)[e]);
+    for (int jj = 1; jj <= l - 1; ++jj) {                          // L29-31
+      cmac(u0, cneg(gam[jj - 1]), v.su(s, k, jj)[e]);
+      cmac(x, cf[jj], rr[jj]);
+    }
+    v.sx(s, k)[e] = x;
+    v.su(s, k, 0)[e] = cmul(cf[l], u0);                            // L34
+  }
+  double2 r0 = rr[0];
+  for (int jj = 1; jj <= l; ++jj) cmac(r0, cneg(gp[jj - 1]), rr[jj]);  // L32
+  v.r(s, 0)[e] = r0;
+}
+
+// convergence test (bicgstab.f90:237,245,296-299) and NaN guard
+__global__ void k_check(int nsys, int iter, double thr, int32_t *__restrict__ active,
+                        const double *__restrict__ norm, int32_t *__restrict__ iters,
+                        int32_t *__restrict__ state) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys || !active[s]) return;
+  double nr = norm[s];
+  if (nr < thr) {
+    active[s] = 0;
+    iters[s] = iter;
+    state[s] = 1;  // converged
+  } else if (isnan(nr)) {
+    active[s] = 0;
+    iters[s] = iter;
+    state[s] = 2;  // NaN: the reference would run to max_iter and report ierr 2
+  }
+}
+
+// init: mu_ij, shift scalars, seed scalars, r_0 = r0~ = b, everything else 0
+__global__ void k_init_scalars(int nsys, int nsh, int l, Scal sc,
+                               const double2 *__restrict__ sigma_all, int nshift_all) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  const double2 one = mk(1, 0);
+  sc.rho[s] = mk(0, 0);
+  sc.rho_old[s] = one;
+  sc.alpha_old[s] = one;
+  sc.alpha[s] = mk(0, 0);
+  sc.omega[s] = one;
+  sc.beta[s] = mk(0, 0);
+  const double2 s1 = sigma_all[(int64_t)s * nshift_all];
+  const int nmu = l * (l + 1) / 2;
+  for (int k = 0; k < nsh; ++k) {
+    int idx = s * nsh + k;
+    sc.inv_phi_old[idx] = one;
+    sc.inv_phi[idx] = one;
+    sc.inv_theta[idx] = one;
+    double2 sg = csub(sigma_all[(int64_t)s * nshift_all + k + 1], s1);
+    sc.sigma[idx] = sg;
+    double2 spow[kMaxL + 1];
+    spow[0] = one;
+    for (int i = 1; i <= l - 1; ++i) spow[i] = cmul(sg, spow[i - 1]);
+    double bin[(kMaxL + 1) * (kMaxL + 2) / 2];
+    for (int jj = 0; jj <= l - 1; ++jj) {
+      int offset = jj * (jj + 1) / 2 + 1;
+      for (int ii = 0; ii <= jj; ++ii) {
+        int ij = offset + ii;
+        bin[ij - 1] = (ii == 0) ? 1.0 : (bin[ij - 2] * (jj - ii + 1)) / ii;
+        sc.mu[(int64_t)idx * nmu + ij - 1] = cmul(mk(bin[ij - 1], 0.0), spow[jj - ii]);
+      }
+    }
+  }
+}
+
+__global__ void k_init_vectors(int nsys, Vecs v, const double2 *__restrict__ rhs, int64_t rhs_ld) {
+  const int s = blockIdx.y;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  double2 b = rhs[(int64_t)s * rhs_ld + e];
+  v.r(s, 0)[e] = b;
+  v.RT[(int64_t)s * v.ld + e] = b;
+}
+
+// x_out[s][k][ldo] <- X (k = 0) or SX (k >= 1)
+__global__ void k_gather_x(Vecs v, double2 *__restrict__ out, int64_t ldo, int64_t n) {
+  const int s = blockIdx.y, k = blockIdx.z;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  double2 val = (k == 0) ? v.x(s)[e] : v.sx(s, k - 1)[e];
+  out[((int64_t)s * (v.nsh + 1) + k) * ldo + e] = val;
+}
+
+__global__ void k_nan_scan(int nsys, int nshift, const double2 *__restrict__ x, int64_t ldo,
+                           int64_t n, int32_t *__restrict__ has_nan) {
+  const int s = blockIdx.x;
+  int found = 0;
+  for (int64_t i = threadIdx.x; i < (int64_t)nshift * n; i += blockDim.x) {
+    int k = (int)(i / n);
+    int64_t e = i % n;
+    double2 v = x[((int64_t)s * nshift + k) * ldo + e];
+    if (isnan(v.x) || isnan(v.y)) found = 1;
+  }
+  if (found) has_nan[s] = 1;
+}
+
+inline dim3 grid1(int n, int bs) { return dim3((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+struct BatchBicgstab::Impl {
+  int nsys, nshift, nsh, l;
+  int64_t ld;
+  DevBuf<double2> R, U, X, RT, SU, SX;
+  DevBuf<double2> scal_seed, scal_shift, mu, coef, gam;
+  DevBuf<double2> sigma_seed;  // per system seed shift sigma_1
+  DevBuf<double> norms;
+  DevBuf<int32_t> active, iters, state;
+  Scal sc{};
+  Vecs vv{};
+  std::vector<double2> h_sigma_seed;
+  std::vector<int32_t> h_active;
+
+  Impl(int nsys_, int nshift_, int l_, int64_t ld_)
+      : nsys(nsys_), nshift(nshift_), nsh(nshift_ - 1), l(l_), ld(ld_) {
+    SGW_REQUIRE(l >= 1 && l <= kMaxL, SGW_E_ARG, "bicgstab: lmax must be in [1, 15]");
+    SGW_REQUIRE(nshift >= 1, SGW_E_ARG, "bicgstab: need at least one shift");
+    const size_t S = nsys;
+    R.alloc(S * (l + 1) * ld);
+    U.alloc(S * (l + 1) * ld);
+    X.alloc(S * ld);
+    RT.alloc(S * ld);
+    if (nsh > 0) {
+      SU.alloc(S * nsh * (l + 1) * ld);
+      SX.alloc(S * nsh * ld);
+    }
+    scal_seed.alloc(S * 7);
+    gam.alloc(S * 3 * l);
+    const size_t nshs = std::max<size_t>(1, S * nsh);
+    scal_shift.alloc(nshs * 10);
+    mu.alloc(nshs * (l * (l + 1) / 2));
+    coef.alloc(nshs * (kMaxL + 2));
+    sigma_seed.alloc(S);
+    norms.alloc(S);
+    active.alloc(S);
+    iters.alloc(S);
+    state.alloc(S);
+    double2 *p = scal_seed.p;
+    sc.rho = p; sc.rho_old = p + S; sc.alpha = p + 2 * S; sc.alpha_old = p + 3 * S;
+    sc.beta = p + 4 * S; sc.omega = p + 5 * S; sc.dot = p + 6 * S;
+    sc.gamma = gam.p; sc.gamma_p = gam.p + S * l; sc.gamma_pp = gam.p + 2 * S * l;
+    double2 *q = scal_shift.p;
+    sc.sigma = q; sc.inv_phi_old = q + nshs; sc.inv_phi = q + 2 * nshs;
+    sc.inv_phi_new = q + 3 * nshs; sc.inv_theta = q + 4 * nshs; sc.alpha_s = q + 5 * nshs;
+    sc.beta_s = q + 6 * nshs; sc.fa = q + 7 * nshs; sc.fb = q + 8 * nshs; sc.ia = q + 9 * nshs;
+    sc.mu = mu.p;
+    sc.coef = coef.p;
+    vv.R = R.p; vv.U = U.p; vv.X = X.p; vv.RT = RT.p; vv.SU = SU.p; vv.SX = SX.p;
+    vv.ld = ld; vv.l = l; vv.nsh = nsh;
+  }
+
+  void sync_active(hipStream_t st) {
+    SGW_HIP(hipMemcpyAsync(h_active.data(), active.p, sizeof(int32_t) * nsys,
+                           hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipStreamSynchronize(st));
+  }
+
+  void op_apply(Operator &op, hipStream_t st, const double2 *in, double2 *out) {
+    op.apply(st, nsys, in, (l + 1) * ld, out, (l + 1) * ld, sigma_seed.p, h_sigma_seed.data(),
+             active.p, h_active.data(), group_dev);
+  }
+
+  const int32_t *group_dev = nullptr;
+
+  void check(hipStream_t st, int iter, double thr) {
+    k_norm<<<nsys, kBlock, 0, st>>>(nsys, active.p, R.p, (l + 1) * ld, ld, norms.p);
+    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, active.p, norms.p, iters.p,
+                                               state.p);
+  }
+};
+
+BatchBicgstab::BatchBicgstab(int nsys, int nshift, int lmax, int64_t ld)
+    : impl_(new Impl(nsys, nshift, lmax, ld)) {}
+BatchBicgstab::~BatchBicgstab() { delete impl_; }
+
+void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, int64_t rhs_ld,
+                          const double2 *d_sigma, const double2 *h_sigma,
+                          const int32_t *d_group, double threshold, int max_iter,
+                          double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
+                          int check_every) {
+  Impl &m = *impl_;
+  const int S = m.nsys, l = m.l;
+  SGW_REQUIRE(d_group != nullptr || op.ngroups() == 1, SGW_E_ARG,
+              "bicgstab: a group index per system is required with several groups");
+  if (d_group) {  // every group index must address an operator group
+    std::vector<int32_t> hg(S);
+    SGW_HIP(hipMemcpyAsync(hg.data(), d_group, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipStreamSynchronize(st));
+    for (int s = 0; s < S; ++s)
+      SGW_REQUIRE(hg[s] >= 0 && hg[s] < op.ngroups(), SGW_E_ARG,
+                  "bicgstab: group index out of range");
+  }
+  m.group_dev = d_group;
+  m.h_sigma_seed.resize(S);
+  for (int s = 0; s < S; ++s) m.h_sigma_seed[s] = h_sigma[(int64_t)s * m.nshift];
+  m.sigma_seed.upload(m.h_sigma_seed.data(), S, st);
+  m.h_active.assign(S, 1);
+  m.active.upload(m.h_active.data(), S, st);
+  SGW_HIP(hipMemsetAsync(m.iters.p, 0, sizeof(int32_t) * S, st));
+  SGW_HIP(hipMemsetAsync(m.state.p, 0, sizeof(int32_t) * S, st));
+  m.R.zero(st);
+  m.U.zero(st);
+  m.X.zero(st);
+  m.SU.zero(st);
+  m.SX.zero(st);
+  const dim3 gv((unsigned)((m.ld + kBlock - 1) / kBlock), S);
+  k_init_vectors<<<gv, kBlock, 0, st>>>(S, m.vv, d_rhs, rhs_ld);
+  k_init_scalars<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, l, m.sc, d_sigma, m.nshift);
+  const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
+  const int64_t sstride = (int64_t)(l + 1) * m.ld;
+  const bool host_op = op.host_side();
+  int iter = 0;
+  int last_iter = 0;
+  for (iter = 1; iter <= max_iter; ++iter) {
+    // ---- bicg_part (Algorithm 3)
+    k_bicg_begin<<<grid1(S, 64), 64, 0, st>>>(S, m.active.p, m.sc);
+    for (int j = 0; j < l; ++j) {
+      {
+        KScope ks_("dotu", st);
+        k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.R.p + j * m.ld, sstride, m.RT.p, m.ld, m.ld,
+                                     m.sc.dot);
+      }
+      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, m.active.p, m.sc);
+      {
+        KScope ks_("bicg_v1", st);
+        k_bicg_v1<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
+      }
+      m.op_apply(op, st, m.U.p + j * m.ld, m.U.p + (j + 1) * m.ld);       // L10
+      {
+        KScope ks_("dotu", st);
+        k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.U.p + (j + 1) * m.ld, sstride, m.RT.p, m.ld,
+                                     m.ld, m.sc.dot);
+      }
+      k_bicg_s2<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, m.active.p, m.sc);
+      {
+        KScope ks_("bicg_v2", st);
+        k_bicg_v2<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
+      }
+      m.op_apply(op, st, m.R.p + j * m.ld, m.R.p + (j + 1) * m.ld);       // L24
+    }
+    m.check(st, iter, threshold);
+    if (host_op) m.sync_active(st);
+    // ---- mr_part (Algorithm 4)
+    {
+      KScope ks_("mr_mgs", st);
+      k_mr_mgs<1024><<<S, 1024, 0, st>>>(m.active.p, m.vv, m.sc, ntau, nmu);
+    }
+    {
+      KScope ks_("mr_v3", st);
+      k_mr_v3<<<gv, kBlock, 0, st>>>(m.active.p, m.vv, m.sc);
+    }
+    m.check(st, iter, threshold);
+    last_iter = iter;
+    if (host_op || (iter % check_every) == 0 || iter == max_iter) {
+      m.sync_active(st);
+      bool any = false;
+      for (int s = 0; s < S; ++s) any |= (m.h_active[s] != 0);
+      if (!any) break;
+    }
+  }
+  (void)last_iter;
+  // systems still active ran out of iterations: ierr 1 (bicgstab.f90:249-253)
+  std::vector<int32_t> st_h(S), it_h(S);
+  SGW_HIP(hipMemcpyAsync(st_h.data(), m.state.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+  SGW_HIP(hipMemcpyAsync(it_h.data(), m.iters.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+  SGW_HIP(hipMemcpyAsync(m.h_active.data(), m.active.p, sizeof(int32_t) * S,
+                         hipMemcpyDeviceToHost, st));
+  // outputs
+  const int64_t nout = std::min<int64_t>(x_ld, m.ld);
+  k_gather_x<<<dim3((unsigned)((nout + kBlock - 1) / kBlock), S, m.nshift), kBlock, 0, st>>>(
+      m.vv, d_x, x_ld, nout);
+  DevBuf<int32_t> nanflag(S);
+  nanflag.zero(st);
+  k_nan_scan<<<S, kBlock, 0, st>>>(S, m.nshift, d_x, x_ld, nout, nanflag.p);
+  std::vector<int32_t> nan_h(S);
+  SGW_HIP(hipMemcpyAsync(nan_h.data(), nanflag.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+  SGW_HIP(hipStreamSynchronize(st));
+  for (int s = 0; s < S; ++s) {
+    int ierr = 0;
+    int its = it_h[s];
+    if (m.h_active[s] || st_h[s] == 2) {
+      ierr = 1;
+      if (m.h_active[s]) its = max_iter + 1;
+    }
+    if (nan_h[s]) ierr = 2;  // bicgstab.f90:264-267
+    if (h_ierr) h_ierr[s] = ierr;
+    if (h_iters) h_iters[s] = its;
+  }
+}
+
+}  // namespace sgw

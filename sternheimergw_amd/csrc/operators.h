@@ -1,0 +1,70 @@
+// Operators (A + sigma) applied by the GPU solvers.
+#pragma once
+
+#include "sgw_internal.h"
+
+namespace sgw {
+
+using host_op_fn = void (*)(const double2 *sigma, const double2 *x, double2 *Ax, int64_t n,
+                            void *ctx);
+
+// A given as a dense column-major n x n matrix: Ax = A x + sigma x.
+class DenseOperator : public Operator {
+ public:
+  DenseOperator(const double2 *h_a, int64_t n);
+  void apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride, double2 *out,
+             int64_t out_stride, const double2 *d_sigma, const double2 *h_sigma,
+             const int32_t *d_active, const int32_t *h_active, const int32_t *d_group) override;
+
+ private:
+  DevBuf<double2> a_;
+};
+
+// Host procedure AA(sigma, x, Ax) of the reference interface.
+class CallbackOperator : public Operator {
+ public:
+  CallbackOperator(host_op_fn fn, void *ctx, int64_t n_) : fn_(fn), ctx_(ctx) { n = n_; ld = n_; }
+  void apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride, double2 *out,
+             int64_t out_stride, const double2 *d_sigma, const double2 *h_sigma,
+             const int32_t *d_active, const int32_t *h_active, const int32_t *d_group) override;
+  bool host_side() const override { return true; }
+
+ private:
+  host_op_fn fn_;
+  void *ctx_;
+  std::vector<double2> hx_, hax_;
+};
+
+struct PwDesc {
+  int n1, n2, n3;
+  const double *vloc;      // n1*n2*n3, x fastest
+  int ngroup;
+  const int32_t *npw;      // [ngroup]
+  int64_t ld;
+  const int32_t *mill;     // [ngroup][ld][3]
+  const double *kin;       // [ngroup][ld] (Ry)
+  int nproj;
+  const double2 *proj;     // [ngroup][nproj][ld]
+  const double *dcoef;     // [ngroup][nproj] (Ry)
+};
+
+class PwOperator : public Operator {
+ public:
+  explicit PwOperator(const PwDesc &desc);
+  void apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride, double2 *out,
+             int64_t out_stride, const double2 *d_sigma, const double2 *h_sigma,
+             const int32_t *d_active, const int32_t *h_active, const int32_t *d_group) override;
+  int64_t group_n(int g) const override;
+  int ngroups() const override { return ngroup_; }
+
+ private:
+  void ensure_scratch(int nvec);
+  int n1_, n2_, n3_, ngroup_, nproj_;
+  int M1_, M2_, nc1_, nc2_, nslot_;
+  std::vector<int32_t> npw_;
+  DevBuf<double> vloc_, kin_, dcoef_;
+  DevBuf<int32_t> d_npw_, col_start_, col_cnt_, sidx_, sm3_, slot_of_, m3_of_;
+  DevBuf<double2> proj_, tw_, S_;
+};
+
+}  // namespace sgw

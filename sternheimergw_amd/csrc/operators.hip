@@ -1,0 +1,404 @@
+// Linear operators applied by the solvers: (A + sigma) x for batches of vectors.
+//
+//  * DenseOperator    A given as a dense column-major matrix (the reference's
+//                     unit-test operator, algo/linear_solver/test/linear_solver.pf:100-113)
+//  * CallbackOperator the reference's procedure argument AA(sigma, xx, Ax)
+//                     (algo/linear_solver/src/select_solver.f90:77-87); runs
+//                     on the host, vectors cross PCIe per call
+//  * PwOperator       the plane-wave Hamiltonian of linear_op
+//                     (algo/linear_solver/src/linear_op.f90:46-206):
+//                     (|k+q+G|^2 + V_loc + sum_p |p> D_p <p| + sigma) x,
+//                     where the projector list holds the nonlocal beta's and,
+//                     with D = alpha_pv, the valence states of P_v.
+#include "operators.h"
+#include "ktimer.h"
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+
+namespace sgw {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// dense
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_dense(int64_t n, const double2 *__restrict__ a,
+                                                  const double2 *__restrict__ in, int64_t sin,
+                                                  double2 *__restrict__ out, int64_t sout,
+                                                  int64_t ld, const double2 *__restrict__ sigma,
+                                                  const int32_t *__restrict__ active) {
+  const int v = blockIdx.y;
+  if (active && !active[v]) return;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ld) return;
+  const double2 *x = in + v * sin;
+  double2 acc = mk(0, 0);
+  if (i < n) {
+    for (int64_t j = 0; j < n; ++j) cmac(acc, a[i + j * n], x[j]);
+    cmac(acc, sigma[v], x[i]);
+  }
+  out[v * sout + i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// plane waves
+// ---------------------------------------------------------------------------
+struct PwDev {
+  int n1, n2, n3;
+  int M1, M2;           // column grid half widths; nc1 = 2M1+1, nc2 = 2M2+1
+  int nc1, nc2, nslot;
+  int64_t ld;
+  int nproj;
+  const double *vloc;          // [n3][n2][n1]
+  const int32_t *npw;          // [ngroup]
+  const int32_t *col_start;    // [ngroup][nslot]
+  const int32_t *col_cnt;      // [ngroup][nslot]
+  const int32_t *sorted_idx;   // [ngroup][ld]
+  const int32_t *sorted_m3;    // [ngroup][ld] (reduced mod n3)
+  const int32_t *slot_of;      // [ngroup][ld]
+  const int32_t *m3_of;        // [ngroup][ld] (reduced mod n3)
+  const double *kin;           // [ngroup][ld]
+  const double2 *proj;         // [ngroup][nproj][ld]
+  const double *dcoef;         // [ngroup][nproj]
+  const double2 *tw1, *tw2, *tw3;  // e^{+2 pi i k/n}
+  double inv_nbox;
+};
+
+// One block per (plane z, vector v): pencil DFTs of one z-plane.
+// Output S[v][slot][z]: the plane's contribution to sum_z (...) before the
+// final z-transform.
+__global__ __launch_bounds__(kBlock) void k_pw_plane(PwDev d, const double2 *__restrict__ in,
+                                                     int64_t sin, const int32_t *__restrict__ group,
+                                                     const int32_t *__restrict__ active,
+                                                     double2 *__restrict__ S) {
+  extern __shared__ double2 smem[];
+  const int z = blockIdx.x, v = blockIdx.y;
+  if (active && !active[v]) return;
+  const int g = group ? group[v] : 0;
+  const int n1 = d.n1, n2 = d.n2, n3 = d.n3, nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot;
+  double2 *w1 = smem;                 // n1
+  double2 *w2 = w1 + n1;              // n2
+  double2 *pin = w2 + n2;             // nslot   (m1, m2)
+  double2 *A = pin + nslot;           // nc1 * n2 (m1, y)
+  double2 *B = A + nc1 * n2;          // n2 * n1  (y, x)
+  for (int i = threadIdx.x; i < n1; i += kBlock) w1[i] = d.tw1[i];
+  for (int i = threadIdx.x; i < n2; i += kBlock) w2[i] = d.tw2[i];
+  const double2 *x = in + v * sin;
+  const int32_t *cs = d.col_start + (int64_t)g * nslot, *cc = d.col_cnt + (int64_t)g * nslot;
+  const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld, *sm3 = d.sorted_m3 + (int64_t)g * d.ld;
+  // z inverse transform of each column at this z
+  for (int c = threadIdx.x; c < nslot; c += kBlock) {
+    double2 acc = mk(0, 0);
+    const int st = cs[c], cnt = cc[c];
+    for (int j = st; j < st + cnt; ++j) {
+      int e = (sm3[j] * z) % n3;
+      cmac(acc, x[sidx[j]], d.tw3[e]);
+    }
+    pin[c] = acc;
+  }
+  __syncthreads();
+  // y inverse: A[m1][y] = sum_m2 pin[m1][m2] e^{+i 2pi m2 y/n2}
+  for (int idx = threadIdx.x; idx < nc1 * n2; idx += kBlock) {
+    const int a = idx / n2, y = idx % n2;
+    double2 acc = mk(0, 0);
+    for (int b = 0; b < nc2; ++b) {
+      int m2 = b - d.M2;
+      int e = ((m2 * y) % n2 + n2) % n2;
+      cmac(acc, pin[a * nc2 + b], w2[e]);
+    }
+    A[idx] = acc;
+  }
+  __syncthreads();
+  // x inverse and local potential: B[y][x] = V(x,y,z) sum_m1 A[m1][y] e^{+i 2pi m1 x/n1}
+  const double *vz = d.vloc + (int64_t)z * n1 * n2;
+  for (int idx = threadIdx.x; idx < n2 * n1; idx += kBlock) {
+    const int y = idx / n1, xx = idx % n1;
+    double2 acc = mk(0, 0);
+    for (int a = 0; a < nc1; ++a) {
+      int m1 = a - d.M1;
+      int e = ((m1 * xx) % n1 + n1) % n1;
+      cmac(acc, A[a * n2 + y], w1[e]);
+    }
+    const double vv = vz[idx];
+    B[idx] = mk(acc.x * vv, acc.y * vv);
+  }
+  __syncthreads();
+  // x forward: A[m1][y] = sum_x B[y][x] e^{-i 2pi m1 x/n1}
+  for (int idx = threadIdx.x; idx < nc1 * n2; idx += kBlock) {
+    const int a = idx / n2, y = idx % n2;
+    const int m1 = a - d.M1;
+    double2 acc = mk(0, 0);
+    for (int xx = 0; xx < n1; ++xx) {
+      int e = ((m1 * xx) % n1 + n1) % n1;
+      cmac(acc, B[y * n1 + xx], cconj(w1[e]));
+    }
+    A[idx] = acc;
+  }
+  __syncthreads();
+  // y forward into the sphere columns
+  double2 *Sv = S + (int64_t)v * nslot * n3;
+  for (int c = threadIdx.x; c < nslot; c += kBlock) {
+    if (cc[c] == 0) continue;
+    const int a = c / nc2, b = c % nc2;
+    const int m2 = b - d.M2;
+    double2 acc = mk(0, 0);
+    for (int y = 0; y < n2; ++y) {
+      int e = ((m2 * y) % n2 + n2) % n2;
+      cmac(acc, A[a * n2 + y], cconj(w2[e]));
+    }
+    Sv[(int64_t)c * n3 + z] = acc;
+  }
+}
+
+__device__ inline double2 wave_sum(double2 v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    v.x += __shfl_xor(v.x, off, 64);
+    v.y += __shfl_xor(v.y, off, 64);
+  }
+  return v;
+}
+
+// One block per vector: z forward transform, kinetic, projectors, shift.
+__global__ __launch_bounds__(kBlock) void k_pw_assemble(PwDev d, const double2 *__restrict__ in,
+                                                        int64_t sin, double2 *__restrict__ out,
+                                                        int64_t sout,
+                                                        const double2 *__restrict__ sigma,
+                                                        const int32_t *__restrict__ group,
+                                                        const int32_t *__restrict__ active,
+                                                        const double2 *__restrict__ S) {
+  extern __shared__ double2 smem[];
+  const int v = blockIdx.x;
+  if (active && !active[v]) return;
+  const int g = group ? group[v] : 0;
+  const int n3 = d.n3, nproj = d.nproj;
+  const int64_t ld = d.ld;
+  double2 *w3 = smem;           // n3
+  double2 *becp = w3 + n3;      // nproj
+  for (int i = threadIdx.x; i < n3; i += kBlock) w3[i] = d.tw3[i];
+  const double2 *x = in + v * sin;
+  const int npw = d.npw[g];
+  const double2 *pg = d.proj + (int64_t)g * nproj * ld;
+  // becp_p = <p|x>, one wave per projector
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int p = wid; p < nproj; p += kBlock / 64) {
+    const double2 *pp = pg + (int64_t)p * ld;
+    double2 acc = mk(0, 0);
+    for (int i = lane; i < npw; i += 64) cmacc(acc, pp[i], x[i]);
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      const double dc = d.dcoef[(int64_t)g * nproj + p];
+      becp[p] = mk(acc.x * dc, acc.y * dc);
+    }
+  }
+  __syncthreads();
+  const double2 sg = sigma[v];
+  const double2 *Sv = S + (int64_t)v * d.nslot * n3;
+  const int32_t *slot = d.slot_of + (int64_t)g * ld, *m3 = d.m3_of + (int64_t)g * ld;
+  const double *kin = d.kin + (int64_t)g * ld;
+  double2 *o = out + v * sout;
+  for (int64_t i = threadIdx.x; i < ld; i += kBlock) {
+    if (i >= npw) {
+      o[i] = mk(0, 0);
+      continue;
+    }
+    const double2 xi = x[i];
+    // kinetic
+    double2 acc = mk(kin[i] * xi.x, kin[i] * xi.y);
+    // local: z forward transform of the column, 1/N
+    const double2 *col = Sv + (int64_t)slot[i] * n3;
+    const int mz = m3[i];
+    double2 loc = mk(0, 0);
+    for (int z = 0; z < n3; ++z) {
+      int e = (mz * z) % n3;
+      cmac(loc, col[z], cconj(w3[e]));
+    }
+    acc.x = fma(loc.x, d.inv_nbox, acc.x);
+    acc.y = fma(loc.y, d.inv_nbox, acc.y);
+    // projectors
+    for (int p = 0; p < nproj; ++p) cmac(acc, pg[(int64_t)p * ld + i], becp[p]);
+    // shift
+    cmac(acc, sg, xi);
+    o[i] = acc;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// DenseOperator
+// ---------------------------------------------------------------------------
+DenseOperator::DenseOperator(const double2 *h_a, int64_t n_) : a_(n_ * n_) {
+  n = n_;
+  ld = n_;
+  a_.upload(h_a, n_ * n_);
+  SGW_HIP(hipDeviceSynchronize());
+}
+
+void DenseOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride,
+                          double2 *out, int64_t out_stride, const double2 *d_sigma,
+                          const double2 *, const int32_t *d_active, const int32_t *,
+                          const int32_t *) {
+  dim3 grid((unsigned)((ld + kBlock - 1) / kBlock), nvec);
+  {
+    KScope ks("dense_matvec", st);
+    k_dense<<<grid, kBlock, 0, st>>>(n, a_.p, in, in_stride, out, out_stride, ld, d_sigma,
+                                     d_active);
+  }
+  SGW_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// CallbackOperator
+// ---------------------------------------------------------------------------
+void CallbackOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride,
+                             double2 *out, int64_t out_stride, const double2 *,
+                             const double2 *h_sigma, const int32_t *, const int32_t *h_active,
+                             const int32_t *) {
+  hx_.resize(n);
+  hax_.resize(n);
+  SGW_HIP(hipStreamSynchronize(st));
+  for (int v = 0; v < nvec; ++v) {
+    if (h_active && !h_active[v]) continue;
+    SGW_HIP(hipMemcpy(hx_.data(), in + v * in_stride, sizeof(double2) * n, hipMemcpyDeviceToHost));
+    fn_(&h_sigma[v], hx_.data(), hax_.data(), n, ctx_);
+    SGW_HIP(hipMemcpy(out + v * out_stride, hax_.data(), sizeof(double2) * n,
+                      hipMemcpyHostToDevice));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// PwOperator
+// ---------------------------------------------------------------------------
+static std::vector<double2> twiddles(int n) {
+  std::vector<double2> w(n);
+  for (int k = 0; k < n; ++k) {
+    double ang = 2.0 * M_PI * (double)k / n;
+    w[k] = mk(std::cos(ang), std::sin(ang));
+  }
+  return w;
+}
+
+static inline int modn(int a, int n) { return ((a % n) + n) % n; }
+
+PwOperator::PwOperator(const PwDesc &desc) {
+  n1_ = desc.n1; n2_ = desc.n2; n3_ = desc.n3;
+  SGW_REQUIRE(n1_ > 0 && n2_ > 0 && n3_ > 0 && n1_ <= 512 && n2_ <= 512 && n3_ <= 512, SGW_E_ARG,
+              "pw operator: FFT box dimensions must be in [1, 512]");
+  ngroup_ = desc.ngroup;
+  SGW_REQUIRE(ngroup_ >= 1, SGW_E_ARG, "pw operator: need at least one group");
+  ld = desc.ld;
+  nproj_ = desc.nproj;
+  SGW_REQUIRE(nproj_ >= 0 && nproj_ <= 4096, SGW_E_ARG, "pw operator: nproj out of range");
+  npw_.assign(desc.npw, desc.npw + ngroup_);
+  int M1 = 0, M2 = 0, M3 = 0;
+  for (int g = 0; g < ngroup_; ++g) {
+    SGW_REQUIRE(npw_[g] >= 1 && npw_[g] <= ld, SGW_E_SHAPE, "pw operator: npw must be in [1, ld]");
+    for (int i = 0; i < npw_[g]; ++i) {
+      const int32_t *m = desc.mill + ((int64_t)g * ld + i) * 3;
+      M1 = std::max(M1, std::abs(m[0]));
+      M2 = std::max(M2, std::abs(m[1]));
+      M3 = std::max(M3, std::abs(m[2]));
+    }
+  }
+  SGW_REQUIRE(2 * M1 + 1 <= n1_ && 2 * M2 + 1 <= n2_ && 2 * M3 + 1 <= n3_, SGW_E_SHAPE,
+              "pw operator: the plane-wave sphere does not fit in the FFT box");
+  n = npw_[0];
+  M1_ = M1; M2_ = M2;
+  nc1_ = 2 * M1 + 1; nc2_ = 2 * M2 + 1;
+  nslot_ = nc1_ * nc2_;
+  // column tables
+  std::vector<int32_t> col_start((size_t)ngroup_ * nslot_, 0), col_cnt((size_t)ngroup_ * nslot_, 0);
+  std::vector<int32_t> sidx((size_t)ngroup_ * ld, 0), sm3((size_t)ngroup_ * ld, 0);
+  std::vector<int32_t> slot_of((size_t)ngroup_ * ld, 0), m3_of((size_t)ngroup_ * ld, 0);
+  for (int g = 0; g < ngroup_; ++g) {
+    std::vector<std::pair<int64_t, int>> key(npw_[g]);
+    std::map<int64_t, int> seen;
+    for (int i = 0; i < npw_[g]; ++i) {
+      const int32_t *m = desc.mill + ((int64_t)g * ld + i) * 3;
+      int slot = (m[0] + M1) * nc2_ + (m[1] + M2);
+      key[i] = {(int64_t)slot * 4096 + (m[2] + 2048), i};
+      SGW_REQUIRE(seen.emplace(key[i].first, i).second, SGW_E_ARG,
+                  "pw operator: duplicate Miller index in a sphere");
+      slot_of[(size_t)g * ld + i] = slot;
+      m3_of[(size_t)g * ld + i] = modn(m[2], n3_);
+    }
+    std::sort(key.begin(), key.end());
+    for (int j = 0; j < npw_[g]; ++j) {
+      int i = key[j].second;
+      int slot = slot_of[(size_t)g * ld + i];
+      if (col_cnt[(size_t)g * nslot_ + slot] == 0) col_start[(size_t)g * nslot_ + slot] = j;
+      col_cnt[(size_t)g * nslot_ + slot]++;
+      sidx[(size_t)g * ld + j] = i;
+      sm3[(size_t)g * ld + j] = m3_of[(size_t)g * ld + i];
+    }
+  }
+  const int64_t nbox = (int64_t)n1_ * n2_ * n3_;
+  vloc_.alloc(nbox); vloc_.upload(desc.vloc, nbox);
+  d_npw_.alloc(ngroup_); d_npw_.upload(npw_.data(), ngroup_);
+  col_start_.alloc(col_start.size()); col_start_.upload(col_start.data(), col_start.size());
+  col_cnt_.alloc(col_cnt.size()); col_cnt_.upload(col_cnt.data(), col_cnt.size());
+  sidx_.alloc(sidx.size()); sidx_.upload(sidx.data(), sidx.size());
+  sm3_.alloc(sm3.size()); sm3_.upload(sm3.data(), sm3.size());
+  slot_of_.alloc(slot_of.size()); slot_of_.upload(slot_of.data(), slot_of.size());
+  m3_of_.alloc(m3_of.size()); m3_of_.upload(m3_of.data(), m3_of.size());
+  kin_.alloc((size_t)ngroup_ * ld); kin_.upload(desc.kin, (size_t)ngroup_ * ld);
+  if (nproj_ > 0) {
+    proj_.alloc((size_t)ngroup_ * nproj_ * ld);
+    proj_.upload(desc.proj, (size_t)ngroup_ * nproj_ * ld);
+    dcoef_.alloc((size_t)ngroup_ * nproj_);
+    dcoef_.upload(desc.dcoef, (size_t)ngroup_ * nproj_);
+  }
+  auto t1 = twiddles(n1_), t2 = twiddles(n2_), t3 = twiddles(n3_);
+  tw_.alloc(n1_ + n2_ + n3_);
+  tw_.upload(t1.data(), n1_);
+  SGW_HIP(hipMemcpy(tw_.p + n1_, t2.data(), sizeof(double2) * n2_, hipMemcpyHostToDevice));
+  SGW_HIP(hipMemcpy(tw_.p + n1_ + n2_, t3.data(), sizeof(double2) * n3_, hipMemcpyHostToDevice));
+  SGW_HIP(hipDeviceSynchronize());
+}
+
+int64_t PwOperator::group_n(int g) const { return npw_.at(g); }
+
+void PwOperator::ensure_scratch(int nvec) {
+  size_t need = (size_t)nvec * nslot_ * n3_;
+  if (S_.n < need) S_.alloc(need);
+}
+
+void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride,
+                       double2 *out, int64_t out_stride, const double2 *d_sigma, const double2 *,
+                       const int32_t *d_active, const int32_t *, const int32_t *d_group) {
+  if (nvec <= 0) return;
+  SGW_REQUIRE(d_group != nullptr || ngroup_ == 1, SGW_E_ARG,
+              "pw operator: a group index per vector is required with several groups");
+  ensure_scratch(nvec);
+  PwDev d;
+  d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
+  d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
+  d.ld = ld; d.nproj = nproj_;
+  d.vloc = vloc_.p; d.npw = d_npw_.p;
+  d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
+  d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
+  d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
+  d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
+  d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
+  size_t lds_plane = sizeof(double2) * (n1_ + n2_ + nslot_ + (size_t)nc1_ * n2_ + (size_t)n2_ * n1_);
+  SGW_REQUIRE(lds_plane <= 160 * 1024, SGW_E_ARG, "pw operator: FFT plane does not fit in LDS");
+  {
+    KScope ks("pw_plane", st);
+    k_pw_plane<<<dim3(n3_, nvec), kBlock, lds_plane, st>>>(d, in, in_stride, d_group, d_active,
+                                                          S_.p);
+  }
+  SGW_HIP(hipGetLastError());
+  size_t lds_asm = sizeof(double2) * (n3_ + nproj_);
+  {
+    KScope ks("pw_assemble", st);
+    k_pw_assemble<<<nvec, kBlock, lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma,
+                                                 d_group, d_active, S_.p);
+  }
+  SGW_HIP(hipGetLastError());
+}
+
+}  // namespace sgw
