@@ -28,7 +28,7 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-fPIC",
-    "-std=c++17",
+    "-std=c++20",
     "-Wall",
     "-Wno-unused-result",
     # never emit packed FP32 (v_pk_*_f32) instructions

@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_norm(int nsys, const int32_t *__rest
 // ---------------------------------------------------------------------------
 struct Scal {
   // seed, per system
-  double2 *rho, *rho_old, *alpha, *alpha_old, *beta, *omega, *dot;
+  double2 *rho, *rho_old, *alpha, *alpha_old, *beta, *omega, *dot, *alpha_new;
   double2 *gamma, *gamma_p, *gamma_pp;  // [S][l]
   // shifts, per (system, shift): index s*nsh + k
   double2 *sigma, *inv_phi_old, *inv_phi, *inv_phi_new, *inv_theta, *alpha_s, *beta_s;
@@ -111,45 +111,51 @@ __global__ void k_bicg_begin(int nsys, const int32_t *__restrict__ active, Scal 
   sc.rho_old[s] = cmul(cneg(sc.omega[s]), sc.rho_old[s]);
 }
 
-// bicgstab.f90:595-599  rho = (r_j, r0~); beta = alpha rho / rho_old; rho_old = rho
-__global__ void k_bicg_s1(int nsys, const int32_t *__restrict__ active, Scal sc) {
+// bicgstab.f90:595-599  rho = (r_j, r0~); beta = alpha rho / rho_old; rho_old = rho.
+// `commit` first completes L18 of the previous step (alpha_old = alpha =
+// alpha_new), which k_bicg_s2 leaves to this kernel so that its per-shift
+// threads can all read the old alpha.
+__global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ active, Scal sc) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys || !active[s]) return;
+  if (commit) {
+    sc.alpha[s] = sc.alpha_new[s];
+    sc.alpha_old[s] = sc.alpha_new[s];
+  }
   double2 rho = sc.dot[s];
   sc.rho[s] = rho;
   sc.beta[s] = cdiv(cmul(sc.alpha[s], rho), sc.rho_old[s]);
   sc.rho_old[s] = rho;
 }
 
-// bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars
+// bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars;
+// one thread per (system, shift), every thread forms the seed alpha itself.
 __global__ void k_bicg_s2(int nsys, int nsh, const int32_t *__restrict__ active, Scal sc) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = t / max(nsh, 1), k = t % max(nsh, 1);
   if (s >= nsys || !active[s]) return;
   const double2 one = mk(1, 0);
-  double2 alpha = cdiv(sc.rho[s], sc.dot[s]);
-  sc.alpha[s] = alpha;
-  double2 beta = sc.beta[s], alpha_old = sc.alpha_old[s];
-  double2 ab = cdiv(cmul(alpha, beta), alpha_old);
-  for (int k = 0; k < nsh; ++k) {
-    int i = s * nsh + k;
-    double2 ipo = sc.inv_phi_old[i], ip = sc.inv_phi[i];
-    double2 t = csub(cdiv(ip, ipo), one);
-    double2 den = cadd(cadd(one, cmul(alpha, sc.sigma[i])), cmul(ab, t));
-    double2 ipn = cdiv(ip, den);                                   // L13
-    double2 q = cdiv(ip, ipo);
-    double2 beta_s = cmul(cmul(q, q), beta);
-    double2 alpha_s = cmul(cdiv(ipn, ip), alpha);
-    double2 th = sc.inv_theta[i];
-    sc.fa[i] = cmul(th, ip);                                       // 1/(theta phi) before L18
-    sc.beta_s[i] = beta_s;
-    sc.alpha_s[i] = alpha_s;
-    sc.inv_phi_new[i] = ipn;
-    sc.inv_phi_old[i] = ip;                                        // L18
-    sc.inv_phi[i] = ipn;
-    sc.fb[i] = cneg(cmul(th, ipn));                                // L26 factor
-    sc.ia[i] = cdiv(one, alpha_s);
-  }
-  sc.alpha_old[s] = alpha;                                         // L18 (seed)
+  const double2 alpha = cdiv(sc.rho[s], sc.dot[s]);
+  if (k == 0) sc.alpha_new[s] = alpha;
+  if (nsh == 0) return;
+  const double2 beta = sc.beta[s], alpha_old = sc.alpha_old[s];
+  const double2 ab = cdiv(cmul(alpha, beta), alpha_old);
+  const int i = s * nsh + k;
+  double2 ipo = sc.inv_phi_old[i], ip = sc.inv_phi[i];
+  double2 q = cdiv(ip, ipo);
+  double2 den = cadd(cadd(one, cmul(alpha, sc.sigma[i])), cmul(ab, csub(q, one)));
+  double2 ipn = cdiv(ip, den);                                     // L13
+  double2 beta_s = cmul(cmul(q, q), beta);
+  double2 alpha_s = cmul(cdiv(ipn, ip), alpha);
+  double2 th = sc.inv_theta[i];
+  sc.fa[i] = cmul(th, ip);                                         // 1/(theta phi) before L18
+  sc.beta_s[i] = beta_s;
+  sc.alpha_s[i] = alpha_s;
+  sc.inv_phi_new[i] = ipn;
+  sc.inv_phi_old[i] = ip;                                          // L18
+  sc.inv_phi[i] = ipn;
+  sc.fb[i] = cneg(cmul(th, ipn));                                  // L26 factor
+  sc.ia[i] = cdiv(one, alpha_s);
 }
 
 // ---------------------------------------------------------------------------
@@ -194,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v2(int j, const int32_t *__rest
   double2 rr[LM];
   for (int i = 0; i <= j; ++i) rr[i] = v.r(s, i)[e];
   // seed residual update r_i -= alpha u_{i+1} (uses the updated u's)
-  const double2 alpha = sc.alpha[s];
+  const double2 alpha = sc.alpha_new[s];
   const double2 malpha = cneg(alpha);
   double2 rnew[LM];
   for (int i = 0; i <= j; ++i) {
@@ -508,7 +514,7 @@ struct BatchBicgstab::Impl {
       SU.alloc(S * nsh * (l + 1) * ld);
       SX.alloc(S * nsh * ld);
     }
-    scal_seed.alloc(S * 7);
+    scal_seed.alloc(S * 8);
     gam.alloc(S * 3 * l);
     const size_t nshs = std::max<size_t>(1, S * nsh);
     scal_shift.alloc(nshs * 10);
@@ -521,7 +527,7 @@ struct BatchBicgstab::Impl {
     state.alloc(S);
     double2 *p = scal_seed.p;
     sc.rho = p; sc.rho_old = p + S; sc.alpha = p + 2 * S; sc.alpha_old = p + 3 * S;
-    sc.beta = p + 4 * S; sc.omega = p + 5 * S; sc.dot = p + 6 * S;
+    sc.beta = p + 4 * S; sc.omega = p + 5 * S; sc.dot = p + 6 * S; sc.alpha_new = p + 7 * S;
     sc.gamma = gam.p; sc.gamma_p = gam.p + S * l; sc.gamma_pp = gam.p + 2 * S * l;
     double2 *q = scal_shift.p;
     sc.sigma = q; sc.inv_phi_old = q + nshs; sc.inv_phi = q + 2 * nshs;
@@ -566,13 +572,16 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   const int S = m.nsys, l = m.l;
   SGW_REQUIRE(d_group != nullptr || op.ngroups() == 1, SGW_E_ARG,
               "bicgstab: a group index per system is required with several groups");
-  if (d_group) {  // every group index must address an operator group
-    std::vector<int32_t> hg(S);
-    SGW_HIP(hipMemcpyAsync(hg.data(), d_group, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-    SGW_HIP(hipStreamSynchronize(st));
+  {  // every group index must address an operator group
+    std::vector<int32_t> hg(S, 0);
+    if (d_group) {
+      SGW_HIP(hipMemcpyAsync(hg.data(), d_group, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+      SGW_HIP(hipStreamSynchronize(st));
+    }
     for (int s = 0; s < S; ++s)
       SGW_REQUIRE(hg[s] >= 0 && hg[s] < op.ngroups(), SGW_E_ARG,
                   "bicgstab: group index out of range");
+    op.prepare(hg.data(), S);
   }
   m.group_dev = d_group;
   m.h_sigma_seed.resize(S);
@@ -604,7 +613,7 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
         k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.R.p + j * m.ld, sstride, m.RT.p, m.ld, m.ld,
                                      m.sc.dot);
       }
-      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, m.active.p, m.sc);
+      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, (iter > 1 || j > 0) ? 1 : 0, m.active.p, m.sc);
       {
         KScope ks_("bicg_v1", st);
         k_bicg_v1<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
@@ -615,7 +624,7 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
         k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.U.p + (j + 1) * m.ld, sstride, m.RT.p, m.ld,
                                      m.ld, m.sc.dot);
       }
-      k_bicg_s2<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, m.active.p, m.sc);
+      k_bicg_s2<<<grid1(S * std::max(m.nsh, 1), 64), 64, 0, st>>>(S, m.nsh, m.active.p, m.sc);
       {
         KScope ks_("bicg_v2", st);
         k_bicg_v2<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);

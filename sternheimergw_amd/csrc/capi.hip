@@ -263,6 +263,10 @@ int sgw_operator_apply_device(sgw_operator *op, int32_t nvec, const int32_t *d_g
     SGW_REQUIRE(op && op->op, SGW_E_ARG, "operator_apply: null operator");
     SGW_REQUIRE(!op->op->host_side(), SGW_E_ARG, "operator_apply_device: host operator");
     SGW_REQUIRE(ldx >= op->op->ld, SGW_E_SHAPE, "operator_apply: ldx < operator ld");
+    std::vector<int32_t> hg(nvec, 0);
+    if (d_group)
+      SGW_HIP(hipMemcpy(hg.data(), d_group, sizeof(int32_t) * nvec, hipMemcpyDeviceToHost));
+    op->op->prepare(hg.data(), nvec);
     op->op->apply((hipStream_t)stream, nvec, D2(d_x), ldx, D2(d_ax), ldx, D2(d_sigma), nullptr,
                   nullptr, nullptr, d_group);
     return 0;
@@ -289,8 +293,10 @@ int sgw_operator_apply(sgw_operator *op, int32_t nvec, const int32_t *group,
     dax.zero(st);
     ds.upload(D2(sigma), nvec, st);
     da.upload(ones.data(), nvec, st);
-    if (group) dg.upload(group, nvec, st);
-    else dg.zero(st);
+    std::vector<int32_t> hg(nvec, 0);
+    if (group) std::copy(group, group + nvec, hg.begin());
+    dg.upload(hg.data(), nvec, st);
+    o.prepare(hg.data(), nvec);
     o.apply(st, nvec, dx.p, ldx, dax.p, ldx, ds.p, D2(sigma), da.p, ones.data(), dg.p);
     SGW_HIP(hipMemcpyAsync(ax, dax.p, sizeof(double2) * nvec * ldx, hipMemcpyDeviceToHost, st));
     SGW_HIP(hipStreamSynchronize(st));
@@ -394,6 +400,13 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
     DevBuf<double2> ax((size_t)nv * ld);
     DevBuf<int32_t> gv(nv), act(nv);
     k_expand_group<<<(nv + 255) / 256, 256, 0, st>>>(nsys, nshift, d_group, gv.p);
+    std::vector<int32_t> hsys(nsys, 0), hv(nv);
+    if (d_group)
+      SGW_HIP(hipMemcpyAsync(hsys.data(), d_group, sizeof(int32_t) * nsys, hipMemcpyDeviceToHost,
+                             st));
+    SGW_HIP(hipStreamSynchronize(st));
+    for (int v = 0; v < nv; ++v) hv[v] = hsys[v / nshift];
+    op->op->prepare(hv.data(), nv);
     std::vector<int32_t> ones(nv, 1);
     act.upload(ones.data(), nv, st);
     op->op->apply(st, nv, D2(d_x), ld, ax.p, ld, D2(d_sigma), nullptr, act.p, nullptr, gv.p);

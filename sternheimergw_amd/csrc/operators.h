@@ -56,15 +56,22 @@ class PwOperator : public Operator {
              const int32_t *d_active, const int32_t *h_active, const int32_t *d_group) override;
   int64_t group_n(int g) const override;
   int ngroups() const override { return ngroup_; }
+  void prepare(const int32_t *h_group, int nvec) override;
 
  private:
   void ensure_scratch(int nvec);
+  // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
+  int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;
+  std::vector<int32_t> plan_key_;
+  DevBuf<int32_t> plan_vec_, plan_cnt_;
+  DevBuf<double2> part_;
   int n1_, n2_, n3_, ngroup_, nproj_;
   int M1_, M2_, nc1_, nc2_, nslot_;
   std::vector<int32_t> npw_;
   DevBuf<double> vloc_, kin_, dcoef_;
-  DevBuf<int32_t> d_npw_, col_start_, col_cnt_, sidx_, sm3_, slot_of_, m3_of_;
-  DevBuf<double2> proj_, tw_, S_;
+  DevBuf<int32_t> d_npw_, col_start_, col_cnt_, col_lo_, sidx_, sm3_, slot_of_, m3_of_;
+  DevBuf<double2> proj_, tw_, S_, T_, loc_;
+  bool columns_contiguous_ = true;
 };
 
 }  // namespace sgw

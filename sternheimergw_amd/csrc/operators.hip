@@ -12,10 +12,12 @@
 //                     with D = alpha_pv, the valence states of P_v.
 #include "operators.h"
 #include "ktimer.h"
+#include "regfft.h"
 
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <type_traits>
 
 namespace sgw {
 
@@ -61,6 +63,7 @@ struct PwDev {
   const int32_t *sorted_m3;    // [ngroup][ld] (reduced mod n3)
   const int32_t *slot_of;      // [ngroup][ld]
   const int32_t *m3_of;        // [ngroup][ld] (reduced mod n3)
+  const int32_t *col_lo;       // [ngroup][nslot] smallest m3 of the column
   const double *kin;           // [ngroup][ld]
   const double2 *proj;         // [ngroup][nproj][ld]
   const double *dcoef;         // [ngroup][nproj]
@@ -151,6 +154,265 @@ __global__ __launch_bounds__(kBlock) void k_pw_plane(PwDev d, const double2 *__r
       cmac(acc, A[a * n2 + y], cconj(w2[e]));
     }
     Sv[(int64_t)c * n3 + z] = acc;
+  }
+}
+
+// z inverse transform of the sphere columns: T[v][z][c] = sum_m3 x(c, m3)
+// e^{+2 pi i m3 z / N}.  One lane per column (m1, m2); the column's plane
+// waves are consecutive in sorted order with m3 = lo .. lo + cnt - 1 (the
+// sphere is convex).
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_zinv(PwDev d, const double2 *__restrict__ in,
+                                                int64_t sin, const int32_t *__restrict__ group,
+                                                const int32_t *__restrict__ active,
+                                                double2 *__restrict__ T) {
+  const int v = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
+  if (active && !active[v]) return;
+  if (c >= d.nslot) return;
+  const int g = group ? group[v] : 0;
+  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  double2 *Tv = T + (int64_t)v * N * d.nslot + c;
+  if (cnt == 0) {
+#pragma unroll
+    for (int z = 0; z < N; ++z) Tv[(int64_t)z * d.nslot] = mk(0, 0);
+    return;
+  }
+  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
+  const double2 *x = in + v * sin;
+  double2 buf[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int m = i < N / 2 ? i : i - N;
+    const int k = m - lo;
+    buf[i] = (k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
+  }
+  reg_fft<N, +1>(buf);
+#pragma unroll
+  for (int z = 0; z < N; ++z) Tv[(int64_t)z * d.nslot] = buf[z];
+}
+
+// z forward transform of the sphere columns back to the plane waves:
+// loc[v][i] = (1/Nbox) sum_z S[v][z][c] e^{-2 pi i m3 z / N} for the plane
+// waves i of column c.
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const double2 *__restrict__ S,
+                                                const int32_t *__restrict__ group,
+                                                const int32_t *__restrict__ active,
+                                                double2 *__restrict__ loc) {
+  const int v = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
+  if (active && !active[v]) return;
+  if (c >= d.nslot) return;
+  const int g = group ? group[v] : 0;
+  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  if (cnt == 0) return;
+  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
+  const double2 *Sv = S + (int64_t)v * N * d.nslot + c;
+  double2 buf[N];
+#pragma unroll
+  for (int z = 0; z < N; ++z) buf[z] = Sv[(int64_t)z * d.nslot];
+  reg_fft<N, -1>(buf);
+  double2 *lv = loc + (int64_t)v * d.ld;
+  const double sc = d.inv_nbox;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int m = i < N / 2 ? i : i - N;
+    const int k = m - lo;
+    if (k >= 0 && k < cnt) lv[sidx[k]] = mk(buf[i].x * sc, buf[i].y * sc);
+  }
+}
+
+// One wavefront per (plane z, vector v), cubic box of compile-time side N.
+// The plane's pencil transforms run as register FFTs (regfft.h):
+//   Z  each lane sums its sphere columns at this z (<= 2M3+1 terms),
+//   Y  lane a < nc1 transforms row m1 = a - M1 over m2 -> y,
+//   X  lane y < N transforms over m1 -> x, multiplies by V(x, y, z) and
+//      transforms back, keeping the nc1 frequencies m1 of the sphere,
+//   Y' lane a < nc1 transforms back over y -> m2 and writes the sphere
+//      columns' values S[v][c][z].
+// LDS holds the column values T[nc1][nc2] and one row buffer P[nc1][N] that
+// the X stage overwrites column by column (each lane owns its column y).
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const double2 *__restrict__ Tin,
+                                                     const int32_t *__restrict__ group,
+                                                     const int32_t *__restrict__ active,
+                                                     double2 *__restrict__ S) {
+  extern __shared__ double2 smem[];
+  const int z = blockIdx.x, v = blockIdx.y;
+  if (active && !active[v]) return;
+  const int g = group ? group[v] : 0;
+  const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M1 = d.M1, M2 = d.M2;
+  const int lane = threadIdx.x;
+  double2 *T = smem;          // nslot
+  double2 *P = T + nslot;     // nc1 * N
+  const int32_t *cc = d.col_cnt + (int64_t)g * nslot;
+  const double2 *Tz = Tin + ((int64_t)v * N + z) * nslot;
+  for (int c = lane; c < nslot; c += 64) T[c] = Tz[c];
+  __syncthreads();
+  double2 buf[N];
+  if (lane < nc1) {  // Y inverse of row a
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      buf[i] = (m >= -M2 && m <= M2) ? T[lane * nc2 + m + M2] : mk(0, 0);
+    }
+    reg_fft<N, +1>(buf);
+#pragma unroll
+    for (int i = 0; i < N; ++i) P[lane * N + i] = buf[i];
+  }
+  __syncthreads();
+  if (lane < N) {  // X inverse, V(r), X forward of column y = lane
+    const int y = lane;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      buf[i] = (m >= -M1 && m <= M1) ? P[(m + M1) * N + y] : mk(0, 0);
+    }
+    reg_fft<N, +1>(buf);
+    const double *vrow = d.vloc + ((int64_t)z * N + y) * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const double vv = vrow[i];
+      buf[i] = mk(buf[i].x * vv, buf[i].y * vv);
+    }
+    reg_fft<N, -1>(buf);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      if (m >= -M1 && m <= M1) P[(m + M1) * N + y] = buf[i];
+    }
+  }
+  __syncthreads();
+  if (lane < nc1) {  // Y forward of row a into the sphere columns
+#pragma unroll
+    for (int i = 0; i < N; ++i) buf[i] = P[lane * N + i];
+    reg_fft<N, -1>(buf);
+    double2 *Sz = S + ((int64_t)v * N + z) * nslot;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      if (m >= -M2 && m <= M2) {
+        const int c = lane * nc2 + m + M2;
+        if (cc[c] > 0) Sz[c] = buf[i];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Projector contraction per k+q group: the vectors of one group share the
+// projector block proj[g] (nproj x npw), so the two complex GEMMs
+//   becp = proj^H X   (nproj x nv, K = npw)   and   HX += proj (D becp)
+// run per (group, chunk of kChunk plane waves, tile of kVT vectors): each
+// block reads its projector chunk once for kVT vectors.
+// ---------------------------------------------------------------------------
+constexpr int kChunk = 64;
+constexpr int kVT = 16;
+constexpr int kPT = 16;  // projectors staged in LDS at a time
+
+// part[((g*tiles + vt)*nchunk + chunk)][p][vl] = sum_{i in chunk} conj(proj_p(i)) x_v(i)
+__global__ __launch_bounds__(256) void k_pw_becp(PwDev d, const double2 *__restrict__ in,
+                                                 int64_t sin, const int32_t *__restrict__ active,
+                                                 const int32_t *__restrict__ gvec,
+                                                 const int32_t *__restrict__ gcnt, int maxper,
+                                                 int tiles, int nchunk,
+                                                 double2 *__restrict__ part) {
+  const int chunk = blockIdx.x, g = blockIdx.y, vt = blockIdx.z;
+  const int cnt = gcnt[g], v0 = vt * kVT;
+  const int npw = d.npw[g], i0 = chunk * kChunk;
+  if (v0 >= cnt || i0 >= npw) return;
+  const int nv = min(kVT, cnt - v0), lim = min(kChunk, npw - i0), nproj = d.nproj;
+  __shared__ double2 xs[kVT][kChunk + 1];
+  __shared__ double2 ps[kPT][kChunk + 1];
+  for (int t = threadIdx.x; t < kVT * kChunk; t += 256) {
+    const int vl = t / kChunk, i = t % kChunk;
+    double2 val = mk(0, 0);
+    if (vl < nv && i < lim) {
+      const int v = gvec[g * maxper + v0 + vl];
+      if (!active || active[v]) val = in[v * sin + i0 + i];
+    }
+    xs[vl][i] = val;
+  }
+  const double2 *pg = d.proj + (int64_t)g * nproj * d.ld + i0;
+  double2 *pout = part + (((int64_t)g * tiles + vt) * nchunk + chunk) * nproj * kVT;
+  for (int p0 = 0; p0 < nproj; p0 += kPT) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kPT * kChunk; t += 256) {
+      const int pl = t / kChunk, i = t % kChunk;
+      ps[pl][i] = (p0 + pl < nproj && i < lim) ? pg[(int64_t)(p0 + pl) * d.ld + i] : mk(0, 0);
+    }
+    __syncthreads();
+    const int o = threadIdx.x;  // kPT * kVT == 256 outputs
+    const int pl = o / kVT, vl = o % kVT;
+    double2 acc = mk(0, 0);
+    for (int i = 0; i < lim; ++i) cmacc(acc, ps[pl][i], xs[vl][i]);
+    if (p0 + pl < nproj) pout[(p0 + pl) * kVT + vl] = acc;
+  }
+}
+
+// out_v = kin x_v + fwfft(V invfft x_v) + proj (D becp_v) + sigma_v x_v for the
+// plane waves of one chunk and the vectors of one tile; becp_v is the sum of
+// the chunk partials, the local term the z forward transform of S.
+__global__ __launch_bounds__(256) void k_pw_assemble2(
+    PwDev d, const double2 *__restrict__ in, int64_t sin, double2 *__restrict__ out, int64_t sout,
+    const double2 *__restrict__ sigma, const int32_t *__restrict__ active,
+    const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
+    int nchunk, const double2 *__restrict__ part, const double2 *__restrict__ S,
+    const double2 *__restrict__ locv) {
+  extern __shared__ double2 smem[];
+  const int chunk = blockIdx.x, g = blockIdx.y, vt = blockIdx.z;
+  const int cnt = gcnt[g], v0 = vt * kVT;
+  if (v0 >= cnt) return;
+  const int nv = min(kVT, cnt - v0), nproj = d.nproj, n3 = d.n3;
+  const int npw = d.npw[g], i0 = chunk * kChunk;
+  const int nck = (npw + kChunk - 1) / kChunk;
+  double2 *becp = smem;            // [nproj][kVT]
+  double2 *w3 = becp + nproj * kVT;  // n3
+  const double2 *pp = part + ((int64_t)g * tiles + vt) * nchunk * nproj * kVT;
+  for (int o = threadIdx.x; o < nproj * kVT; o += 256) {
+    double2 acc = mk(0, 0);
+    for (int c = 0; c < nck; ++c) {
+      const double2 t = pp[(int64_t)c * nproj * kVT + o];
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    const double dc = d.dcoef[(int64_t)g * nproj + o / kVT];
+    becp[o] = mk(acc.x * dc, acc.y * dc);
+  }
+  for (int i = threadIdx.x; i < n3; i += 256) w3[i] = d.tw3[i];
+  __syncthreads();
+  const int i = threadIdx.x % kChunk, vq = threadIdx.x / kChunk;  // 4 vector lanes
+  const int64_t e = (int64_t)i0 + i;
+  if (e >= d.ld) return;
+  const double2 *pg = d.proj + (int64_t)g * nproj * d.ld;
+  const int64_t ge = (int64_t)g * d.ld + e;
+  for (int vl = vq; vl < nv; vl += 256 / kChunk) {
+    const int v = gvec[g * maxper + v0 + vl];
+    if (active && !active[v]) continue;
+    double2 *o = out + v * sout;
+    if (e >= npw) {
+      o[e] = mk(0, 0);
+      continue;
+    }
+    const double2 xi = in[v * sin + e];
+    const double kin = d.kin[ge];
+    double2 acc = mk(kin * xi.x, kin * xi.y);
+    if (locv) {  // z transform done by k_pw_zfwd (already scaled)
+      const double2 l = locv[v * d.ld + e];
+      acc.x += l.x;
+      acc.y += l.y;
+    } else {
+      const double2 *col = S + ((int64_t)v * d.nslot + d.slot_of[ge]) * n3;
+      const int mz = d.m3_of[ge];
+      double2 loc = mk(0, 0);
+      for (int z = 0; z < n3; ++z) cmac(loc, col[z], cconj(w3[(mz * z) % n3]));
+      acc.x = fma(loc.x, d.inv_nbox, acc.x);
+      acc.y = fma(loc.y, d.inv_nbox, acc.y);
+    }
+    for (int p = 0; p < nproj; ++p) cmac(acc, pg[(int64_t)p * d.ld + e], becp[p * kVT + vl]);
+    cmac(acc, sigma[v], xi);
+    o[e] = acc;
   }
 }
 
@@ -314,6 +576,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
   std::vector<int32_t> col_start((size_t)ngroup_ * nslot_, 0), col_cnt((size_t)ngroup_ * nslot_, 0);
   std::vector<int32_t> sidx((size_t)ngroup_ * ld, 0), sm3((size_t)ngroup_ * ld, 0);
   std::vector<int32_t> slot_of((size_t)ngroup_ * ld, 0), m3_of((size_t)ngroup_ * ld, 0);
+  std::vector<int32_t> col_lo((size_t)ngroup_ * nslot_, 0);
   for (int g = 0; g < ngroup_; ++g) {
     std::vector<std::pair<int64_t, int>> key(npw_[g]);
     std::map<int64_t, int> seen;
@@ -330,7 +593,13 @@ PwOperator::PwOperator(const PwDesc &desc) {
     for (int j = 0; j < npw_[g]; ++j) {
       int i = key[j].second;
       int slot = slot_of[(size_t)g * ld + i];
-      if (col_cnt[(size_t)g * nslot_ + slot] == 0) col_start[(size_t)g * nslot_ + slot] = j;
+      const int m3raw = desc.mill[((int64_t)g * ld + i) * 3 + 2];
+      if (col_cnt[(size_t)g * nslot_ + slot] == 0) {
+        col_start[(size_t)g * nslot_ + slot] = j;
+        col_lo[(size_t)g * nslot_ + slot] = m3raw;
+      } else if (m3raw != col_lo[(size_t)g * nslot_ + slot] + col_cnt[(size_t)g * nslot_ + slot]) {
+        columns_contiguous_ = false;  // a column with a gap in m3: no z-FFT kernels
+      }
       col_cnt[(size_t)g * nslot_ + slot]++;
       sidx[(size_t)g * ld + j] = i;
       sm3[(size_t)g * ld + j] = m3_of[(size_t)g * ld + i];
@@ -345,6 +614,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
   sm3_.alloc(sm3.size()); sm3_.upload(sm3.data(), sm3.size());
   slot_of_.alloc(slot_of.size()); slot_of_.upload(slot_of.data(), slot_of.size());
   m3_of_.alloc(m3_of.size()); m3_of_.upload(m3_of.data(), m3_of.size());
+  col_lo_.alloc(col_lo.size()); col_lo_.upload(col_lo.data(), col_lo.size());
   kin_.alloc((size_t)ngroup_ * ld); kin_.upload(desc.kin, (size_t)ngroup_ * ld);
   if (nproj_ > 0) {
     proj_.alloc((size_t)ngroup_ * nproj_ * ld);
@@ -365,6 +635,8 @@ int64_t PwOperator::group_n(int g) const { return npw_.at(g); }
 void PwOperator::ensure_scratch(int nvec) {
   size_t need = (size_t)nvec * nslot_ * n3_;
   if (S_.n < need) S_.alloc(need);
+  if (T_.n < need) T_.alloc(need);
+  if (loc_.n < (size_t)nvec * ld) loc_.alloc((size_t)nvec * ld);
 }
 
 void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride,
@@ -381,24 +653,97 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   d.vloc = vloc_.p; d.npw = d_npw_.p;
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
+  d.col_lo = col_lo_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
   d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
-  size_t lds_plane = sizeof(double2) * (n1_ + n2_ + nslot_ + (size_t)nc1_ * n2_ + (size_t)n2_ * n1_);
-  SGW_REQUIRE(lds_plane <= 160 * 1024, SGW_E_ARG, "pw operator: FFT plane does not fit in LDS");
-  {
+  const bool cubic = n1_ == n2_ && n2_ == n3_ && columns_contiguous_;
+  const size_t lds_fft = sizeof(double2) * (nslot_ + (size_t)nc1_ * n1_);
+  const dim3 gcol((unsigned)((nslot_ + 63) / 64), nvec);
+  const double2 *locv = nullptr;
+  auto launch_fft = [&](auto tag) {
+    constexpr int NB = decltype(tag)::value;
+    {
+      KScope ks("pw_zinv", st);
+      k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T_.p);
+    }
+    {
+      KScope ks("pw_plane", st);
+      k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, T_.p, d_group, d_active, S_.p);
+    }
+    {
+      KScope ks("pw_zfwd", st);
+      k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p);
+    }
+    locv = loc_.p;
+  };
+  if (cubic && n1_ == 24) {
+    launch_fft(std::integral_constant<int, 24>{});
+  } else if (cubic && n1_ == 32) {
+    launch_fft(std::integral_constant<int, 32>{});
+  } else if (cubic && n1_ == 36) {
+    launch_fft(std::integral_constant<int, 36>{});
+  } else if (cubic && n1_ == 48) {
+    launch_fft(std::integral_constant<int, 48>{});
+  } else if (cubic && n1_ == 64) {
+    launch_fft(std::integral_constant<int, 64>{});
+  } else {
+    size_t lds_plane =
+        sizeof(double2) * (n1_ + n2_ + nslot_ + (size_t)nc1_ * n2_ + (size_t)n2_ * n1_);
+    SGW_REQUIRE(lds_plane <= 64 * 1024, SGW_E_ARG, "pw operator: FFT plane does not fit in LDS");
     KScope ks("pw_plane", st);
     k_pw_plane<<<dim3(n3_, nvec), kBlock, lds_plane, st>>>(d, in, in_stride, d_group, d_active,
                                                           S_.p);
   }
   SGW_HIP(hipGetLastError());
-  size_t lds_asm = sizeof(double2) * (n3_ + nproj_);
-  {
-    KScope ks("pw_assemble", st);
-    k_pw_assemble<<<nvec, kBlock, lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma,
-                                                 d_group, d_active, S_.p);
+  SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
+              "pw operator: prepare() was not called for this batch of vectors");
+  const int nchunk = (int)((ld + kChunk - 1) / kChunk);
+  if (nproj_ > 0) {
+    KScope ks("pw_becp", st);
+    k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
+        d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
+        part_.p);
   }
   SGW_HIP(hipGetLastError());
+  {
+    const size_t lds_asm = sizeof(double2) * ((size_t)nproj_ * kVT + n3_);
+    KScope ks("pw_assemble", st);
+    k_pw_assemble2<<<dim3(nchunk, ngroup_, plan_tiles_), 256, lds_asm, st>>>(
+        d, in, in_stride, out, out_stride, d_sigma, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
+        plan_tiles_, nchunk, part_.p, S_.p, locv);
+  }
+  SGW_HIP(hipGetLastError());
+}
+
+void PwOperator::prepare(const int32_t *h_group, int nvec) {
+  std::vector<int32_t> key(h_group ? h_group : nullptr, h_group ? h_group + nvec : nullptr);
+  if (!h_group) key.assign(nvec, 0);
+  if (plan_nvec_ == nvec && key == plan_key_) return;
+  std::vector<std::vector<int32_t>> lists(ngroup_);
+  for (int v = 0; v < nvec; ++v) {
+    SGW_REQUIRE(key[v] >= 0 && key[v] < ngroup_, SGW_E_ARG, "pw operator: group index out of range");
+    lists[key[v]].push_back(v);
+  }
+  int maxper = 1;
+  for (auto &l : lists) maxper = std::max<int>(maxper, (int)l.size());
+  std::vector<int32_t> vec((size_t)ngroup_ * maxper, 0), cnt(ngroup_);
+  for (int g = 0; g < ngroup_; ++g) {
+    cnt[g] = (int32_t)lists[g].size();
+    std::copy(lists[g].begin(), lists[g].end(), vec.begin() + (size_t)g * maxper);
+  }
+  plan_vec_.alloc(vec.size());
+  plan_vec_.upload(vec.data(), vec.size());
+  plan_cnt_.alloc(cnt.size());
+  plan_cnt_.upload(cnt.data(), cnt.size());
+  plan_max_ = maxper;
+  plan_tiles_ = (maxper + kVT - 1) / kVT;
+  const int nchunk = (int)((ld + kChunk - 1) / kChunk);
+  const size_t need = (size_t)ngroup_ * plan_tiles_ * nchunk * std::max(nproj_, 1) * kVT;
+  if (part_.n < need) part_.alloc(need);
+  SGW_HIP(hipDeviceSynchronize());
+  plan_key_ = std::move(key);
+  plan_nvec_ = nvec;
 }
 
 }  // namespace sgw

@@ -150,6 +150,10 @@ struct Operator {
                      const double2 *h_sigma, const int32_t *d_active,
                      const int32_t *h_active, const int32_t *d_group) = 0;
   // number of valid entries of vector v's group (padding beyond is zero)
+  // Called with the host copy of the per-vector group indices before a
+  // sequence of apply() calls on nvec vectors (a solve); operators that work
+  // on groups of vectors build their plans here.
+  virtual void prepare(const int32_t *h_group, int nvec) { (void)h_group; (void)nvec; }
   virtual int64_t group_n(int g) const { (void)g; return n; }
   virtual int ngroups() const { return 1; }
   virtual bool host_side() const { return false; }

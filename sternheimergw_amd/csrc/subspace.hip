@@ -173,6 +173,7 @@ int SubspaceSolver::solve(Operator &op, hipStream_t st, int32_t group, const dou
   DevBuf<int32_t> dgroup(1), dactive(1), dnan(1);
   int32_t one = 1;
   dgroup.upload(&group, 1, st);
+  op.prepare(&group, 1);
   dactive.upload(&one, 1, st);
   dnan.zero(st);
   // abs_threshold = threshold * norm(b)   (linear_solver.f90:200-215)

@@ -1,0 +1,84 @@
+"""Text summaries of rocprofv3 results (rocpd SQLite databases).
+
+    python -m sternheimergw_amd.profiling trace <run_results.db>
+        per-kernel launches, total/average/min/max duration and time share
+    python -m sternheimergw_amd.profiling pmc <run_results.db> [copy-bytes]
+        per-kernel average counter value (e.g. FETCH_SIZE or WRITE_SIZE, KiB
+        per dispatch); with copy-bytes, the largest copy kernel's reading is
+        used to scale KiB into bytes (calibration against a copy of known size)
+"""
+from __future__ import annotations
+
+import collections
+import sqlite3
+import sys
+
+
+def _short(name: str) -> str:
+    name = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    depth, cut = 0, len(name)
+    for i, ch in enumerate(name):  # drop the parameter list
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            cut = i
+            break
+    name = name[:cut]
+    return name if len(name) < 90 else name[:87] + "..."
+
+
+def trace(db: str) -> str:
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, duration from kernels").fetchall()
+    agg = collections.defaultdict(list)
+    for name, dur in rows:
+        agg[name].append(dur)
+    total = sum(sum(v) for v in agg.values())
+    out = ["| kernel | launches | total ms | avg us | min us | max us | share |",
+           "|---|---|---|---|---|---|---|"]
+    for name, durs in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        t = sum(durs)
+        out.append(f"| `{_short(name)}` | {len(durs)} | {t / 1e6:.3f} | {t / len(durs) / 1e3:.2f} | "
+                   f"{min(durs) / 1e3:.2f} | {max(durs) / 1e3:.2f} | {t / total:.3f} |")
+    return "\n".join(out)
+
+
+def pmc(db: str, copy_bytes: int = 0) -> str:
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value, grid_size from counters_collection"
+                       ).fetchall()
+    agg = collections.defaultdict(list)
+    counter = None
+    for name, cname, val, grid in rows:
+        agg[name].append(val)
+        counter = cname
+    scale = None
+    if copy_bytes:
+        copies = [(max(v), n) for n, v in agg.items() if "copy" in n.lower() or "elementwise" in n.lower()]
+        if copies:
+            kib, _ = max(copies)
+            scale = copy_bytes / (kib * 1024.0) if kib else None
+    out = [f"counter: {counter}" + (f"; copy calibration factor {scale:.4f} (bytes per KiB*1024)" if scale else ""),
+           "| kernel | dispatches | avg KiB | max KiB | avg bytes (calibrated) |", "|---|---|---|---|---|"]
+    for name, vals in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+        avg = sum(vals) / len(vals)
+        cal = f"{avg * 1024 * scale:.4g}" if scale else "-"
+        out.append(f"| `{_short(name)}` | {len(vals)} | {avg:.1f} | {max(vals):.1f} | {cal} |")
+    return "\n".join(out)
+
+
+def main(argv):
+    if len(argv) < 3 or argv[1] not in ("trace", "pmc"):
+        print(__doc__)
+        return 2
+    if argv[1] == "trace":
+        print(trace(argv[2]))
+    else:
+        print(pmc(argv[2], int(argv[3]) if len(argv) > 3 else 0))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))
