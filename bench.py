@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--nshift", type=int, default=32)
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample of the batch (0 to skip)")
+    ap.add_argument("--kernel-timing", type=int, default=1,
+                    help="record HIP events around every timed kernel launch (0 to skip)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (MiB) for counter calibration")
     return ap.parse_args()
@@ -144,7 +146,7 @@ def main():
         step()
         print(f"warmup {i}: max iterations {iters.max()}, ierr max {ierr.max()}", flush=True)
     torch.cuda.synchronize()
-    _lib.check(L.sgw_kernel_timing(1))
+    _lib.check(L.sgw_kernel_timing(args.kernel_timing))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     total_vec_happly = 0
@@ -157,7 +159,8 @@ def main():
     elapsed = time.perf_counter() - t0
     timings = {}
     happly_kernels = ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble")
-    for name in happly_kernels + ("bicg_v1", "bicg_v2", "mr_mgs", "mr_v3", "dotu"):
+    for name in happly_kernels + ("bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed",
+                                  "dotu"):
         ms, cnt = ctypes.c_double(), ctypes.c_int64()
         _lib.check(L.sgw_kernel_timing_query(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         timings[name] = (ms.value, cnt.value)

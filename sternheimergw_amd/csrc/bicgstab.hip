@@ -27,6 +27,7 @@ namespace sgw {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kShiftChunk = 8;  // shifted systems per block of k_shift_cycle
 
 // ---------------------------------------------------------------------------
 // reductions
@@ -130,7 +131,10 @@ __global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ acti
 
 // bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars;
 // one thread per (system, shift), every thread forms the seed alpha itself.
-__global__ void k_bicg_s2(int nsys, int nsh, const int32_t *__restrict__ active, Scal sc) {
+// The per-step factors go to slot (s*nsh + k)*l + j: the shifted vectors are
+// updated once per iteration, for all l steps (k_shift_cycle).
+__global__ void k_bicg_s2(int nsys, int nsh, int l, int j, const int32_t *__restrict__ active,
+                          Scal sc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = t / max(nsh, 1), k = t % max(nsh, 1);
   if (s >= nsys || !active[s]) return;
@@ -148,23 +152,30 @@ __global__ void k_bicg_s2(int nsys, int nsh, const int32_t *__restrict__ active,
   double2 beta_s = cmul(cmul(q, q), beta);
   double2 alpha_s = cmul(cdiv(ipn, ip), alpha);
   double2 th = sc.inv_theta[i];
-  sc.fa[i] = cmul(th, ip);                                         // 1/(theta phi) before L18
-  sc.beta_s[i] = beta_s;
-  sc.alpha_s[i] = alpha_s;
+  const int pj = i * l + j;
+  sc.fa[pj] = cmul(th, ip);                                        // 1/(theta phi) before L18
+  sc.beta_s[pj] = beta_s;
+  sc.alpha_s[pj] = alpha_s;
   sc.inv_phi_new[i] = ipn;
   sc.inv_phi_old[i] = ip;                                          // L18
   sc.inv_phi[i] = ipn;
-  sc.fb[i] = cneg(cmul(th, ipn));                                  // L26 factor
-  sc.ia[i] = cdiv(one, alpha_s);
+  sc.fb[pj] = cneg(cmul(th, ipn));                                 // L26 factor
+  sc.ia[pj] = cdiv(one, alpha_s);
 }
 
 // ---------------------------------------------------------------------------
 // element-wise vector kernels; grid = (ceil(ld/kBlock), S)
 // ---------------------------------------------------------------------------
+// residual snapshot slots: r_i at the start of BiCG step j (i <= j), then
+// r_{l-1} after the last step
+__host__ __device__ constexpr int snap_index(int j, int i) { return j * (j + 1) / 2 + i; }
+__host__ __device__ constexpr int snap_final(int l) { return l * (l + 1) / 2; }
+
 struct Vecs {
-  double2 *R, *U, *X, *RT, *SU, *SX;
+  double2 *R, *U, *X, *RT, *SU, *SX, *RS;
   int64_t ld;
-  int l, nsh;
+  int l, nsh, nsnap;
+  __device__ double2 *rs(int s, int q) const { return RS + ((int64_t)s * nsnap + q) * ld; }
   __device__ double2 *r(int s, int i) const { return R + ((int64_t)s * (l + 1) + i) * ld; }
   __device__ double2 *u(int s, int i) const { return U + ((int64_t)s * (l + 1) + i) * ld; }
   __device__ double2 *x(int s) const { return X + (int64_t)s * ld; }
@@ -189,58 +200,132 @@ __global__ __launch_bounds__(kBlock) void k_bicg_v1(int j, const int32_t *__rest
   }
 }
 
-// L14-L19 (shifts), L21-L22 (seed residuals), L25 (seed x), L26 (shifts).
-__global__ __launch_bounds__(kBlock) void k_bicg_v2(int j, const int32_t *__restrict__ active,
-                                                    Vecs v, Scal sc) {
+// Seed half of step j after the matrix-vector product: L21-L22 (seed
+// residuals r_i -= alpha u_{i+1}) and L25 (x += alpha u_0).  The residuals
+// r_0..r_j before the update are kept as snapshots for the shifted systems
+// (their L15/L19 use them); after the last step, r_{l-1} after its update
+// (used by L26 of that step) is kept as well.
+__global__ __launch_bounds__(kBlock) void k_bicg_seed(int j, const int32_t *__restrict__ active,
+                                                      Vecs v, Scal sc) {
   const int s = blockIdx.y;
   if (!active[s]) return;
   const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (e >= v.ld) return;
-  constexpr int LM = kMaxL + 1;
-  double2 rr[LM];
-  for (int i = 0; i <= j; ++i) rr[i] = v.r(s, i)[e];
-  // seed residual update r_i -= alpha u_{i+1} (uses the updated u's)
   const double2 alpha = sc.alpha_new[s];
   const double2 malpha = cneg(alpha);
-  double2 rnew[LM];
   for (int i = 0; i <= j; ++i) {
-    double2 t = rr[i];
+    double2 t = v.r(s, i)[e];
+    v.rs(s, snap_index(j, i))[e] = t;
     cmac(t, malpha, v.u(s, i + 1)[e]);
-    rnew[i] = t;
     v.r(s, i)[e] = t;
+    if (j == v.l - 1 && i == j) v.rs(s, snap_final(v.l))[e] = t;
   }
-  // L25 x += alpha u_0
-  {
-    double2 *px = v.x(s) + e;
-    double2 t = *px;
-    cmac(t, alpha, v.u(s, 0)[e]);
-    *px = t;
-  }
-  for (int k = 0; k < v.nsh; ++k) {
+  double2 *px = v.x(s) + e;
+  double2 t = *px;
+  cmac(t, alpha, v.u(s, 0)[e]);
+  *px = t;
+}
+
+// All updates of the shifted systems in one iteration, one pass over their
+// vectors: the l BiCG steps (L14-L19 and L26 of Algorithm 3, with the seed
+// residual snapshots of each step) and, for systems still running after the
+// BiCG half (mr_active), the MR step (L28-L34 of Algorithm 4, with the
+// Gram-Schmidt residuals before the delayed r_0 update).  Every u_i^sigma
+// and x^sigma is read and written once per iteration.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restrict__ active,
+                                                        const int32_t *__restrict__ mr_active,
+                                                        Vecs v, Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  constexpr int l = L;  // compile-time degree: every array below stays in registers
+  constexpr int nsnap = L * (L + 1) / 2 + 1;
+  const bool do_mr = mr_active[s] != 0;
+  double2 rs[nsnap];
+#pragma unroll
+  for (int q = 0; q < nsnap; ++q) rs[q] = v.rs(s, q)[e];
+  double2 rr[L + 1];
+#pragma unroll
+  for (int i = 0; i <= l; ++i) rr[i] = do_mr ? v.r(s, i)[e] : mk(0, 0);
+  const double2 *gam = sc.gamma + s * l;
+  const int k0 = blockIdx.z * kShiftChunk, k1 = min(v.nsh, k0 + kShiftChunk);
+  for (int k = k0; k < k1; ++k) {
     const int idx = s * v.nsh + k;
-    const double2 mbs = cneg(sc.beta_s[idx]);
-    const double2 fa = sc.fa[idx];
-    double2 u0 = mk(0, 0);
-    for (int i = 0; i <= j; ++i) {                       // L15
-      double2 *pu = v.su(s, k, i) + e;
-      double2 t = cmul(mbs, *pu);
-      cmac(t, fa, rr[i]);
-      *pu = t;
-      if (i == 0) u0 = t;
+    double2 u[L + 1];
+#pragma unroll
+    for (int i = 0; i <= l; ++i) u[i] = v.su(s, k, i)[e];
+    double2 x = v.sx(s, k)[e];
+    const double2 msg = cneg(sc.sigma[idx]);
+#pragma unroll
+    for (int j = 0; j < l; ++j) {
+      const int pj = idx * l + j;
+      const double2 mbs = cneg(sc.beta_s[pj]), fa = sc.fa[pj];
+#pragma unroll
+      for (int i = 0; i <= j; ++i) {                                 // L15
+        double2 t = cmul(mbs, u[i]);
+        cmac(t, fa, rs[snap_index(j, i)]);
+        u[i] = t;
+      }
+      cmac(x, sc.alpha_s[pj], u[0]);                                 // L17
+      // L19 then L26: u_{j+1} = (fa r_j + fb r_j') / alpha^sigma - sigma u_j,
+      // r_j' = r_j after the seed update of step j
+      const double2 rjn = (j + 1 < l) ? rs[snap_index(j + 1, j)] : rs[snap_final(l)];
+      double2 uj1 = cmul(fa, rs[snap_index(j, j)]);
+      cmac(uj1, sc.fb[pj], rjn);
+      uj1 = cmul(sc.ia[pj], uj1);
+      cmac(uj1, msg, u[j]);
+      u[j + 1] = uj1;
     }
-    {                                                    // L17
-      double2 *px = v.sx(s, k) + e;
-      double2 t = *px;
-      cmac(t, sc.alpha_s[idx], u0);
-      *px = t;
+    if (do_mr) {
+      const double2 *cf = sc.coef + (int64_t)idx * (kMaxL + 2);
+      cmac(x, cf[0], rr[0]);                                         // L28
+      double2 u0 = u[0];
+      cmac(u0, cneg(gam[l - 1]), u[l]);
+#pragma unroll
+      for (int jj = 1; jj <= l - 1; ++jj) {                          // L29-31
+        cmac(u0, cneg(gam[jj - 1]), u[jj]);
+        cmac(x, cf[jj], rr[jj]);
+      }
+      u[0] = cmul(cf[l], u0);                                        // L34
     }
-    // L19 then L26: u_{j+1} = ((fa r_j) + fb r_j_new) / alpha^s - sigma u_j
-    double2 uj1 = cmul(fa, rr[j]);
-    cmac(uj1, sc.fb[idx], rnew[j]);
-    uj1 = cmul(sc.ia[idx], uj1);
-    cmac(uj1, cneg(sc.sigma[idx]), v.su(s, k, j)[e]);
-    v.su(s, k, j + 1)[e] = uj1;
+#pragma unroll
+    for (int i = 0; i <= l; ++i) v.su(s, k, i)[e] = u[i];
+    v.sx(s, k)[e] = x;
   }
+}
+
+// Algorithm 4 seed updates: x, u_0 (L14-17) and the delayed r_0 (L32),
+// bicgstab.f90:833-846, 919-925.  Runs after k_shift_cycle, which needs r_0
+// before the delayed update.
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_mr_seed(const int32_t *__restrict__ active, Vecs v,
+                                                    Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  constexpr int l = L;
+  double2 rr[L + 1];
+#pragma unroll
+  for (int i = 0; i <= l; ++i) rr[i] = v.r(s, i)[e];
+  const double2 *gam = sc.gamma + s * l, *gp = sc.gamma_p + s * l, *gpp = sc.gamma_pp + s * l;
+  double2 x = v.x(s)[e];
+  cmac(x, gam[0], rr[0]);                                            // L14
+  double2 u0 = v.u(s, 0)[e];
+  cmac(u0, cneg(gam[l - 1]), v.u(s, l)[e]);
+#pragma unroll
+  for (int jj = 1; jj <= l - 1; ++jj) {                              // L15-17
+    cmac(u0, cneg(gam[jj - 1]), v.u(s, jj)[e]);
+    cmac(x, gpp[jj - 1], rr[jj]);
+  }
+  v.x(s)[e] = x;
+  v.u(s, 0)[e] = u0;
+  double2 r0 = rr[0];
+#pragma unroll
+  for (int jj = 1; jj <= l; ++jj) cmac(r0, cneg(gp[jj - 1]), rr[jj]);  // L32
+  v.r(s, 0)[e] = r0;
 }
 
 // Algorithm 4, MGS part and all scalars (one block per system).
@@ -353,65 +438,31 @@ __global__ __launch_bounds__(NT) void k_mr_mgs(const int32_t *__restrict__ activ
   (void)ntau;
 }
 
-// Algorithm 4 vector updates: seed x, u_0 (L14-17), shifted x, u_0 (L28-34),
-// delayed r_0 update (L32).  bicgstab.f90:833-925.
-__global__ __launch_bounds__(kBlock) void k_mr_v3(const int32_t *__restrict__ active, Vecs v,
-                                                  Scal sc) {
-  const int s = blockIdx.y;
-  if (!active[s]) return;
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (e >= v.ld) return;
-  const int l = v.l;
-  double2 rr[kMaxL + 1];
-  for (int i = 0; i <= l; ++i) rr[i] = v.r(s, i)[e];
-  const double2 *gam = sc.gamma + s * l, *gp = sc.gamma_p + s * l, *gpp = sc.gamma_pp + s * l;
-  {
-    double2 x = v.x(s)[e];
-    cmac(x, gam[0], rr[0]);                                        // L14
-    double2 u0 = v.u(s, 0)[e];
-    cmac(u0, cneg(gam[l - 1]), v.u(s, l)[e]);
-    for (int jj = 1; jj <= l - 1; ++jj) {                          // L15-17
-      cmac(u0, cneg(gam[jj - 1]), v.u(s, jj)[e]);
-      cmac(x, gpp[jj - 1], rr[jj]);
-    }
-    v.x(s)[e] = x;
-    v.u(s, 0)[e] = u0;
-  }
-  for (int k = 0; k < v.nsh; ++k) {
-    const int idx = s * v.nsh + k;
-    const double2 *cf = sc.coef + (int64_t)idx * (kMaxL + 2);
-    double2 x = v.sx(s, k)[e];
-    cmac(x, cf[0], rr[0]);                                         // L28
-    double2 u0 = v.su(s, k, 0)[e];
-    cmac(u0, cneg(gam[l - 1]), v.su(s, k, l)[e]);
-    for (int jj = 1; jj <= l - 1; ++jj) {                          // L29-31
-      cmac(u0, cneg(gam[jj - 1]), v.su(s, k, jj)[e]);
-      cmac(x, cf[jj], rr[jj]);
-    }
-    v.sx(s, k)[e] = x;
-    v.su(s, k, 0)[e] = cmul(cf[l], u0);                            // L34
-  }
-  double2 r0 = rr[0];
-  for (int jj = 1; jj <= l; ++jj) cmac(r0, cneg(gp[jj - 1]), rr[jj]);  // L32
-  v.r(s, 0)[e] = r0;
-}
-
-// convergence test (bicgstab.f90:237,245,296-299) and NaN guard
-__global__ void k_check(int nsys, int iter, double thr, int32_t *__restrict__ active,
-                        const double *__restrict__ norm, int32_t *__restrict__ iters,
-                        int32_t *__restrict__ state) {
+// Convergence test (bicgstab.f90:237,245,296-299) and NaN guard on the
+// systems in `in`: out[s] = in[s] and not converged.  After the BiCG half
+// `out` is mr_active (the MR half runs only for those); after the MR half
+// `in` is mr_active and `out` the active set of the next iteration.
+__global__ void k_check(int nsys, int iter, double thr, const int32_t *__restrict__ in,
+                        int32_t *__restrict__ out, const double *__restrict__ norm,
+                        int32_t *__restrict__ iters, int32_t *__restrict__ state) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys || !active[s]) return;
+  if (s >= nsys) return;
+  if (!in[s]) {
+    out[s] = 0;
+    return;
+  }
   double nr = norm[s];
+  int keep = 1;
   if (nr < thr) {
-    active[s] = 0;
+    keep = 0;
     iters[s] = iter;
     state[s] = 1;  // converged
   } else if (isnan(nr)) {
-    active[s] = 0;
+    keep = 0;
     iters[s] = iter;
     state[s] = 2;  // NaN: the reference would run to max_iter and report ierr 2
   }
+  out[s] = keep;
 }
 
 // init: mu_ij, shift scalars, seed scalars, r_0 = r0~ = b, everything else 0
@@ -488,18 +539,49 @@ inline dim3 grid1(int n, int bs) { return dim3((n + bs - 1) / bs); }
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
+// launch a kernel templated on the BiCGStab degree l (1..kMaxL)
+template <template <int> class K, typename... Args>
+void launch_l(int l, dim3 grid, dim3 block, hipStream_t st, Args... args) {
+  switch (l) {
+#define SGW_L(n) \
+  case n: K<n>::launch(grid, block, st, args...); break;
+    SGW_L(1) SGW_L(2) SGW_L(3) SGW_L(4) SGW_L(5) SGW_L(6) SGW_L(7) SGW_L(8)
+    SGW_L(9) SGW_L(10) SGW_L(11) SGW_L(12) SGW_L(13) SGW_L(14) SGW_L(15)
+#undef SGW_L
+    default: throw Error{SGW_E_ARG, "bicgstab: lmax must be in [1, 15]"};
+  }
+}
+template <int L> struct ShiftCycle {
+  template <typename... A> static void launch(dim3 g, dim3 b, hipStream_t st, A... a) {
+    k_shift_cycle<L><<<g, b, 0, st>>>(a...);
+  }
+};
+template <int L> struct MrSeed {
+  template <typename... A> static void launch(dim3 g, dim3 b, hipStream_t st, A... a) {
+    k_mr_seed<L><<<g, b, 0, st>>>(a...);
+  }
+};
+
 struct BatchBicgstab::Impl {
   int nsys, nshift, nsh, l;
   int64_t ld;
-  DevBuf<double2> R, U, X, RT, SU, SX;
+  DevBuf<double2> R, U, X, RT, SU, SX, RS;
   DevBuf<double2> scal_seed, scal_shift, mu, coef, gam;
   DevBuf<double2> sigma_seed;  // per system seed shift sigma_1
   DevBuf<double> norms;
-  DevBuf<int32_t> active, iters, state;
+  DevBuf<int32_t> active, mr_active, iters, state;
   Scal sc{};
   Vecs vv{};
   std::vector<double2> h_sigma_seed;
   std::vector<int32_t> h_active;
+  int32_t *pinned_active = nullptr;  // 2 x nsys flags for the lagged convergence test
+  hipEvent_t flag_ev[2] = {nullptr, nullptr};
+
+  ~Impl() {
+    if (pinned_active) (void)hipHostFree(pinned_active);
+    for (auto &e : flag_ev)
+      if (e) (void)hipEventDestroy(e);
+  }
 
   Impl(int nsys_, int nshift_, int l_, int64_t ld_)
       : nsys(nsys_), nshift(nshift_), nsh(nshift_ - 1), l(l_), ld(ld_) {
@@ -514,15 +596,18 @@ struct BatchBicgstab::Impl {
       SU.alloc(S * nsh * (l + 1) * ld);
       SX.alloc(S * nsh * ld);
     }
+    const int nsnap = snap_final(l) + 1;
+    RS.alloc(S * nsnap * ld);
     scal_seed.alloc(S * 8);
     gam.alloc(S * 3 * l);
     const size_t nshs = std::max<size_t>(1, S * nsh);
-    scal_shift.alloc(nshs * 10);
+    scal_shift.alloc(nshs * 5 + nshs * l * 5);  // state + per-step factors
     mu.alloc(nshs * (l * (l + 1) / 2));
     coef.alloc(nshs * (kMaxL + 2));
     sigma_seed.alloc(S);
     norms.alloc(S);
     active.alloc(S);
+    mr_active.alloc(S);
     iters.alloc(S);
     state.alloc(S);
     double2 *p = scal_seed.p;
@@ -531,12 +616,18 @@ struct BatchBicgstab::Impl {
     sc.gamma = gam.p; sc.gamma_p = gam.p + S * l; sc.gamma_pp = gam.p + 2 * S * l;
     double2 *q = scal_shift.p;
     sc.sigma = q; sc.inv_phi_old = q + nshs; sc.inv_phi = q + 2 * nshs;
-    sc.inv_phi_new = q + 3 * nshs; sc.inv_theta = q + 4 * nshs; sc.alpha_s = q + 5 * nshs;
-    sc.beta_s = q + 6 * nshs; sc.fa = q + 7 * nshs; sc.fb = q + 8 * nshs; sc.ia = q + 9 * nshs;
+    sc.inv_phi_new = q + 3 * nshs; sc.inv_theta = q + 4 * nshs;
+    double2 *qs = q + 5 * nshs;
+    const size_t nstep = nshs * l;
+    sc.alpha_s = qs; sc.beta_s = qs + nstep; sc.fa = qs + 2 * nstep; sc.fb = qs + 3 * nstep;
+    sc.ia = qs + 4 * nstep;
     sc.mu = mu.p;
     sc.coef = coef.p;
-    vv.R = R.p; vv.U = U.p; vv.X = X.p; vv.RT = RT.p; vv.SU = SU.p; vv.SX = SX.p;
-    vv.ld = ld; vv.l = l; vv.nsh = nsh;
+    vv.R = R.p; vv.U = U.p; vv.X = X.p; vv.RT = RT.p; vv.SU = SU.p; vv.SX = SX.p; vv.RS = RS.p;
+    vv.ld = ld; vv.l = l; vv.nsh = nsh; vv.nsnap = nsnap;
+    SGW_HIP(hipHostMalloc((void **)&pinned_active, sizeof(int32_t) * 2 * S, hipHostMallocDefault));
+    SGW_HIP(hipEventCreateWithFlags(&flag_ev[0], hipEventDisableTiming));
+    SGW_HIP(hipEventCreateWithFlags(&flag_ev[1], hipEventDisableTiming));
   }
 
   void sync_active(hipStream_t st) {
@@ -552,9 +643,10 @@ struct BatchBicgstab::Impl {
 
   const int32_t *group_dev = nullptr;
 
-  void check(hipStream_t st, int iter, double thr) {
-    k_norm<<<nsys, kBlock, 0, st>>>(nsys, active.p, R.p, (l + 1) * ld, ld, norms.p);
-    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, active.p, norms.p, iters.p,
+  // ||r_0|| test on the systems of `in`; survivors go to `out`
+  void check(hipStream_t st, int iter, double thr, int32_t *in, int32_t *out) {
+    k_norm<<<nsys, kBlock, 0, st>>>(nsys, in, R.p, (l + 1) * ld, ld, norms.p);
+    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, in, out, norms.p, iters.p,
                                                state.p);
   }
 };
@@ -624,31 +716,56 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
         k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.U.p + (j + 1) * m.ld, sstride, m.RT.p, m.ld,
                                      m.ld, m.sc.dot);
       }
-      k_bicg_s2<<<grid1(S * std::max(m.nsh, 1), 64), 64, 0, st>>>(S, m.nsh, m.active.p, m.sc);
+      k_bicg_s2<<<grid1(S * std::max(m.nsh, 1), 64), 64, 0, st>>>(S, m.nsh, l, j, m.active.p,
+                                                                   m.sc);
       {
-        KScope ks_("bicg_v2", st);
-        k_bicg_v2<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
+        KScope ks_("bicg_seed", st);
+        k_bicg_seed<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
       }
       m.op_apply(op, st, m.R.p + j * m.ld, m.R.p + (j + 1) * m.ld);       // L24
     }
-    m.check(st, iter, threshold);
-    if (host_op) m.sync_active(st);
-    // ---- mr_part (Algorithm 4)
+    m.check(st, iter, threshold, m.active.p, m.mr_active.p);
+    // ---- mr_part (Algorithm 4) on the systems still running
     {
       KScope ks_("mr_mgs", st);
-      k_mr_mgs<1024><<<S, 1024, 0, st>>>(m.active.p, m.vv, m.sc, ntau, nmu);
+      k_mr_mgs<1024><<<S, 1024, 0, st>>>(m.mr_active.p, m.vv, m.sc, ntau, nmu);
+    }
+    if (m.nsh > 0) {  // the shifted systems: BiCG half of this iteration and, if running, MR half
+      KScope ks_("shift_cycle", st);
+      const dim3 gs(gv.x, gv.y, (unsigned)((m.nsh + kShiftChunk - 1) / kShiftChunk));
+      launch_l<ShiftCycle>(l, gs, dim3(kBlock), st, (const int32_t *)m.active.p,
+                           (const int32_t *)m.mr_active.p, m.vv, m.sc);
     }
     {
-      KScope ks_("mr_v3", st);
-      k_mr_v3<<<gv, kBlock, 0, st>>>(m.active.p, m.vv, m.sc);
+      KScope ks_("mr_seed", st);
+      launch_l<MrSeed>(l, gv, dim3(kBlock), st, (const int32_t *)m.mr_active.p, m.vv, m.sc);
     }
-    m.check(st, iter, threshold);
+    m.check(st, iter, threshold, m.mr_active.p, m.active.p);
     last_iter = iter;
-    if (host_op || (iter % check_every) == 0 || iter == max_iter) {
+    if (host_op) {  // host operators read the active flags on every call
       m.sync_active(st);
       bool any = false;
       for (int s = 0; s < S; ++s) any |= (m.h_active[s] != 0);
       if (!any) break;
+    } else {
+      // Lagged test: copy this iteration's flags to pinned memory and wait
+      // only for the previous iteration's copy, so the next iteration is
+      // already queued while the host looks.  An iteration enqueued after
+      // every system has stopped is a no-op (all kernels skip inactive
+      // systems).
+      const int slot = iter & 1;
+      SGW_HIP(hipMemcpyAsync(m.pinned_active + (size_t)slot * S, m.active.p,
+                             sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+      SGW_HIP(hipEventRecord(m.flag_ev[slot], st));
+      const bool last = iter == max_iter;
+      if (iter >= 2 || last) {
+        const int prev = last ? slot : (slot ^ 1);
+        SGW_HIP(hipEventSynchronize(m.flag_ev[prev]));
+        bool any = false;
+        const int32_t *pa = m.pinned_active + (size_t)prev * S;
+        for (int s = 0; s < S; ++s) any |= (pa[s] != 0);
+        if (!any) break;
+      }
     }
   }
   (void)last_iter;

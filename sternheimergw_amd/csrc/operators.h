@@ -64,7 +64,7 @@ class PwOperator : public Operator {
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;
   std::vector<int32_t> plan_key_;
   DevBuf<int32_t> plan_vec_, plan_cnt_;
-  DevBuf<double2> part_;
+  DevBuf<double2> part_, becp_;
   int n1_, n2_, n3_, ngroup_, nproj_;
   int M1_, M2_, nc1_, nc2_, nslot_;
   std::vector<int32_t> npw_;

@@ -311,6 +311,42 @@ constexpr int kChunk = 64;
 constexpr int kVT = 16;
 constexpr int kPT = 16;  // projectors staged in LDS at a time
 
+// does any vector of the tile still run?  (uniform across the block)
+__device__ inline bool tile_active(const int32_t *__restrict__ active,
+                                   const int32_t *__restrict__ vecs, int nv) {
+  int any = 0;
+  for (int k = 0; k < nv; ++k) any |= active[vecs[k]];
+  return any != 0;
+}
+
+// becp[g][vt][p][vl] = D_p sum over the chunk partials (once per tile)
+__global__ __launch_bounds__(256) void k_pw_becp_reduce(PwDev d,
+                                                        const int32_t *__restrict__ active,
+                                                        const int32_t *__restrict__ gvec,
+                                                        const int32_t *__restrict__ gcnt,
+                                                        int maxper, int tiles, int nchunk,
+                                                        const double2 *__restrict__ part,
+                                                        double2 *__restrict__ becp) {
+  const int g = blockIdx.x, vt = blockIdx.y;
+  const int cnt = gcnt[g], v0 = vt * kVT;
+  if (v0 >= cnt) return;
+  const int nv = min(kVT, cnt - v0), nproj = d.nproj;
+  if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
+  const int nck = (d.npw[g] + kChunk - 1) / kChunk;
+  const double2 *pp = part + ((int64_t)g * tiles + vt) * nchunk * nproj * kVT;
+  double2 *out = becp + ((int64_t)g * tiles + vt) * nproj * kVT;
+  for (int o = threadIdx.x; o < nproj * kVT; o += 256) {
+    double2 acc = mk(0, 0);
+    for (int c = 0; c < nck; ++c) {
+      const double2 t = pp[(int64_t)c * nproj * kVT + o];
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    const double dc = d.dcoef[(int64_t)g * nproj + o / kVT];
+    out[o] = mk(acc.x * dc, acc.y * dc);
+  }
+}
+
 // part[((g*tiles + vt)*nchunk + chunk)][p][vl] = sum_{i in chunk} conj(proj_p(i)) x_v(i)
 __global__ __launch_bounds__(256) void k_pw_becp(PwDev d, const double2 *__restrict__ in,
                                                  int64_t sin, const int32_t *__restrict__ active,
@@ -323,6 +359,7 @@ __global__ __launch_bounds__(256) void k_pw_becp(PwDev d, const double2 *__restr
   const int npw = d.npw[g], i0 = chunk * kChunk;
   if (v0 >= cnt || i0 >= npw) return;
   const int nv = min(kVT, cnt - v0), lim = min(kChunk, npw - i0), nproj = d.nproj;
+  if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   __shared__ double2 xs[kVT][kChunk + 1];
   __shared__ double2 ps[kPT][kChunk + 1];
   for (int t = threadIdx.x; t < kVT * kChunk; t += 256) {
@@ -366,53 +403,68 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   if (v0 >= cnt) return;
   const int nv = min(kVT, cnt - v0), nproj = d.nproj, n3 = d.n3;
   const int npw = d.npw[g], i0 = chunk * kChunk;
-  const int nck = (npw + kChunk - 1) / kChunk;
+  if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   double2 *becp = smem;            // [nproj][kVT]
   double2 *w3 = becp + nproj * kVT;  // n3
-  const double2 *pp = part + ((int64_t)g * tiles + vt) * nchunk * nproj * kVT;
-  for (int o = threadIdx.x; o < nproj * kVT; o += 256) {
-    double2 acc = mk(0, 0);
-    for (int c = 0; c < nck; ++c) {
-      const double2 t = pp[(int64_t)c * nproj * kVT + o];
-      acc.x += t.x;
-      acc.y += t.y;
-    }
-    const double dc = d.dcoef[(int64_t)g * nproj + o / kVT];
-    becp[o] = mk(acc.x * dc, acc.y * dc);
-  }
+  const double2 *bsrc = part + ((int64_t)g * tiles + vt) * nproj * kVT;  // reduced D becp
+  for (int o = threadIdx.x; o < nproj * kVT; o += 256) becp[o] = bsrc[o];
   for (int i = threadIdx.x; i < n3; i += 256) w3[i] = d.tw3[i];
   __syncthreads();
-  const int i = threadIdx.x % kChunk, vq = threadIdx.x / kChunk;  // 4 vector lanes
+  // thread = (plane wave i of the chunk, vector quarter vq): vectors
+  // vl = vq + 4k, k < 4, accumulate together so each projector value is
+  // loaded once per thread
+  constexpr int kVQ = 256 / kChunk;
+  constexpr int kPer = kVT / kVQ;
+  const int i = threadIdx.x % kChunk, vq = threadIdx.x / kChunk;
   const int64_t e = (int64_t)i0 + i;
   if (e >= d.ld) return;
   const double2 *pg = d.proj + (int64_t)g * nproj * d.ld;
   const int64_t ge = (int64_t)g * d.ld + e;
-  for (int vl = vq; vl < nv; vl += 256 / kChunk) {
-    const int v = gvec[g * maxper + v0 + vl];
-    if (active && !active[v]) continue;
-    double2 *o = out + v * sout;
-    if (e >= npw) {
-      o[e] = mk(0, 0);
-      continue;
-    }
-    const double2 xi = in[v * sin + e];
-    const double kin = d.kin[ge];
-    double2 acc = mk(kin * xi.x, kin * xi.y);
+  int vv[kPer];
+  bool on[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int vl = vq + kVQ * k;
+    on[k] = vl < nv;
+    vv[k] = on[k] ? gvec[g * maxper + v0 + vl] : 0;
+    if (on[k] && active && !active[vv[k]]) on[k] = false;
+  }
+  if (e >= npw) {  // padding stays zero
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+      if (on[k]) out[vv[k] * sout + e] = mk(0, 0);
+    return;
+  }
+  const double kin = d.kin[ge];
+  double2 xi[kPer], acc[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    xi[k] = on[k] ? in[vv[k] * sin + e] : mk(0, 0);
+    acc[k] = mk(kin * xi[k].x, kin * xi[k].y);
+    if (!on[k]) continue;
     if (locv) {  // z transform done by k_pw_zfwd (already scaled)
-      const double2 l = locv[v * d.ld + e];
-      acc.x += l.x;
-      acc.y += l.y;
+      const double2 l = locv[vv[k] * d.ld + e];
+      acc[k].x += l.x;
+      acc[k].y += l.y;
     } else {
-      const double2 *col = S + ((int64_t)v * d.nslot + d.slot_of[ge]) * n3;
+      const double2 *col = S + ((int64_t)vv[k] * d.nslot + d.slot_of[ge]) * n3;
       const int mz = d.m3_of[ge];
       double2 loc = mk(0, 0);
       for (int z = 0; z < n3; ++z) cmac(loc, col[z], cconj(w3[(mz * z) % n3]));
-      acc.x = fma(loc.x, d.inv_nbox, acc.x);
-      acc.y = fma(loc.y, d.inv_nbox, acc.y);
+      acc[k].x = fma(loc.x, d.inv_nbox, acc[k].x);
+      acc[k].y = fma(loc.y, d.inv_nbox, acc[k].y);
     }
-    for (int p = 0; p < nproj; ++p) cmac(acc, pg[(int64_t)p * d.ld + e], becp[p * kVT + vl]);
-    cmac(acc, sigma[v], xi);
-    o[e] = acc;
+  }
+  for (int p = 0; p < nproj; ++p) {
+    const double2 pv = pg[(int64_t)p * d.ld + e];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) cmac(acc[k], pv, becp[p * kVT + vq + kVQ * k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (!on[k]) continue;
+    cmac(acc[k], sigma[vv[k]], xi[k]);
+    out[vv[k] * sout + e] = acc[k];
   }
 }
 
@@ -704,6 +756,8 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
         d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
         part_.p);
+    k_pw_becp_reduce<<<dim3(ngroup_, plan_tiles_), 256, 0, st>>>(
+        d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk, part_.p, becp_.p);
   }
   SGW_HIP(hipGetLastError());
   {
@@ -711,7 +765,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     KScope ks("pw_assemble", st);
     k_pw_assemble2<<<dim3(nchunk, ngroup_, plan_tiles_), 256, lds_asm, st>>>(
         d, in, in_stride, out, out_stride, d_sigma, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
-        plan_tiles_, nchunk, part_.p, S_.p, locv);
+        plan_tiles_, nchunk, becp_.p, S_.p, locv);
   }
   SGW_HIP(hipGetLastError());
 }
@@ -741,6 +795,10 @@ void PwOperator::prepare(const int32_t *h_group, int nvec) {
   const int nchunk = (int)((ld + kChunk - 1) / kChunk);
   const size_t need = (size_t)ngroup_ * plan_tiles_ * nchunk * std::max(nproj_, 1) * kVT;
   if (part_.n < need) part_.alloc(need);
+  const size_t needb = (size_t)ngroup_ * plan_tiles_ * std::max(nproj_, 1) * kVT;
+  if (becp_.n < needb) becp_.alloc(needb);
+  if (nproj_ == 0) becp_.zero();
+  ensure_scratch(nvec);
   SGW_HIP(hipDeviceSynchronize());
   plan_key_ = std::move(key);
   plan_nvec_ = nvec;
