@@ -28,6 +28,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 RES_TARGET = 1e-8
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "happly_traffic.json")
 
 
 def parse():
@@ -43,7 +44,10 @@ def parse():
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample of the batch (0 to skip)")
     ap.add_argument("--kernel-timing", type=int, default=1,
-                    help="record HIP events around every timed kernel launch (0 to skip)")
+                    help="HIP-event timing level in the timed steps: 0 off, 1 one event pair "
+                         "per H-apply, 2 one per kernel launch")
+    ap.add_argument("--breakdown", type=int, default=1,
+                    help="after the timed steps, one untimed step with per-kernel events")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (MiB) for counter calibration")
     return ap.parse_args()
@@ -157,14 +161,33 @@ def main():
               f"{iters.max()}", flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    timings = {}
-    happly_kernels = ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble")
-    for name in happly_kernels + ("bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed",
-                                  "dotu"):
-        ms, cnt = ctypes.c_double(), ctypes.c_int64()
-        _lib.check(L.sgw_kernel_timing_query(name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
-        timings[name] = (ms.value, cnt.value)
+
+    def query(names):
+        res = {}
+        for name in names:
+            ms, cnt = ctypes.c_double(), ctypes.c_int64()
+            _lib.check(L.sgw_kernel_timing_query(name.encode(), ctypes.byref(ms),
+                                                 ctypes.byref(cnt)))
+            res[name] = (ms.value, cnt.value)
+        return res
+
+    happly = query(("happly",))["happly"]
     _lib.check(L.sgw_kernel_timing(0))
+    happly_kernels = ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble")
+    solver_kernels = ("bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed", "dotu")
+    breakdown = None
+    if args.breakdown:
+        # per-kernel split from one extra, untimed step (events around every launch)
+        _lib.check(L.sgw_kernel_timing(2))
+        tb = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        tb = time.perf_counter() - tb
+        per = query(happly_kernels + solver_kernels)
+        _lib.check(L.sgw_kernel_timing(0))
+        breakdown = {k: {"share": round(v[0] / 1e3 / tb, 4),
+                         "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2), "launches": v[1]}
+                     for k, v in per.items()}
 
     # true residuals of every (system, shift) in FP64
     d_res = torch.zeros((nsys, nshift), dtype=torch.float64, device=dev)
@@ -179,16 +202,29 @@ def main():
     good = int((res <= RES_TARGET).sum())
     solves_per_s = good * args.steps / elapsed
 
-    # roofline of the H-apply (pw_plane + pw_assemble)
+    # roofline of the H-apply (one event pair around its kernels on the solver stream)
     nbox = int(np.prod(w.box))
     npw_sys = w.npw[w.group]
     mean_npw = float(npw_sys.mean())
     bh = algorithmic_bytes_per_happly(nbox, mean_npw, w.nproj, args.nbnd)
     fh = flops_per_happly(nbox, mean_npw, w.nproj)
-    t_h = sum(timings[k][0] for k in happly_kernels) / 1e3
-    launches = timings["pw_plane"][1]
+    t_h = happly[0] / 1e3
+    launches = happly[1]
     achieved = bh * total_vec_happly / t_h / 1e9 if t_h > 0 else 0.0
-    kernel_share = {k: round(v[0] / 1e3 / elapsed, 4) for k, v in timings.items()}
+
+    traffic, traffic_note = None, None
+    if os.path.exists(TRAFFIC_FILE):
+        # HBM bytes per H-apply from two rocprofv3 --pmc runs of this command
+        # (FETCH_SIZE, WRITE_SIZE; each scaled by a plain copy of known size)
+        with open(TRAFFIC_FILE) as f:
+            prof = json.load(f)
+        if prof.get("workload") == [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold]:
+            traffic = prof["traffic_bytes_per_happly"]
+            traffic_note = {"source": "profiles/happly_traffic.json",
+                            "algorithmic_bytes_per_launch": round(bh * total_vec_happly
+                                                                  / max(launches, 1)),
+                            "traffic_over_algorithmic": round(
+                                traffic / (bh * total_vec_happly / max(launches, 1)), 3)}
 
     out = {
         "metric": "Sternheimer solves/s at 1e-8 residual (48^3 FFT, 32 shifts); % HBM roofline",
@@ -214,21 +250,25 @@ def main():
             "setup_s": round(setup_s, 2),
         },
         "roofline": {
-            "kernel": "H-apply: " + "+".join(happly_kernels),
+            "kernel": "H-apply (" + "+".join(happly_kernels) + ")",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": 8000.0,
             "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_detail": traffic_note,
             "algorithmic_bytes_per_vector": round(bh),
             "vector_happlies": total_vec_happly,
             "launches": launches,
+            "vectors_per_launch": round(total_vec_happly / max(launches, 1), 1),
             "avg_launch_ms": round(t_h * 1e3 / max(launches, 1), 4),
+            "happly_time_share": round(t_h / elapsed, 4),
             "fp64_tflops_equiv": round(fh * total_vec_happly / t_h / 1e12, 2) if t_h else 0.0,
         },
-        "kernel_time_share": kernel_share,
     }
+    if breakdown is not None:
+        out["kernel_breakdown_untimed_step"] = breakdown
     if args.copy_calibration_mib:
         # a plain copy of known size: its FETCH_SIZE/WRITE_SIZE counters scale
         # the H-apply's counters into bytes

@@ -198,10 +198,11 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
                               double *d_res, void *stream);
 
 /* Kernel timing with HIP events recorded on each kernel's launch stream.
- * sgw_kernel_timing(1) enables and resets, (0) disables and resets; query
- * returns the summed duration (ms) and launch count of one kernel name
- * ("pw_plane", "pw_assemble", "bicg_v1", "bicg_v2", "mr_mgs", "mr_v3",
- * "dotu", "dense_matvec"). */
+ * sgw_kernel_timing(level) sets the level and resets the totals: 0 off,
+ * 1 one event pair per operator application ("happly"), 2 also one pair per
+ * kernel launch ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble",
+ * "bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed", "dotu").
+ * Query returns the summed duration (ms) and launch count of one name. */
 int sgw_kernel_timing(int enable);
 int sgw_kernel_timing_query(const char *name, double *total_ms, int64_t *launches);
 

@@ -701,25 +701,25 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
     k_bicg_begin<<<grid1(S, 64), 64, 0, st>>>(S, m.active.p, m.sc);
     for (int j = 0; j < l; ++j) {
       {
-        KScope ks_("dotu", st);
+        KScope ks_("dotu", st, 2);
         k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.R.p + j * m.ld, sstride, m.RT.p, m.ld, m.ld,
                                      m.sc.dot);
       }
       k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, (iter > 1 || j > 0) ? 1 : 0, m.active.p, m.sc);
       {
-        KScope ks_("bicg_v1", st);
+        KScope ks_("bicg_v1", st, 2);
         k_bicg_v1<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
       }
       m.op_apply(op, st, m.U.p + j * m.ld, m.U.p + (j + 1) * m.ld);       // L10
       {
-        KScope ks_("dotu", st);
+        KScope ks_("dotu", st, 2);
         k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.U.p + (j + 1) * m.ld, sstride, m.RT.p, m.ld,
                                      m.ld, m.sc.dot);
       }
       k_bicg_s2<<<grid1(S * std::max(m.nsh, 1), 64), 64, 0, st>>>(S, m.nsh, l, j, m.active.p,
                                                                    m.sc);
       {
-        KScope ks_("bicg_seed", st);
+        KScope ks_("bicg_seed", st, 2);
         k_bicg_seed<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
       }
       m.op_apply(op, st, m.R.p + j * m.ld, m.R.p + (j + 1) * m.ld);       // L24
@@ -727,17 +727,17 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
     m.check(st, iter, threshold, m.active.p, m.mr_active.p);
     // ---- mr_part (Algorithm 4) on the systems still running
     {
-      KScope ks_("mr_mgs", st);
+      KScope ks_("mr_mgs", st, 2);
       k_mr_mgs<1024><<<S, 1024, 0, st>>>(m.mr_active.p, m.vv, m.sc, ntau, nmu);
     }
     if (m.nsh > 0) {  // the shifted systems: BiCG half of this iteration and, if running, MR half
-      KScope ks_("shift_cycle", st);
+      KScope ks_("shift_cycle", st, 2);
       const dim3 gs(gv.x, gv.y, (unsigned)((m.nsh + kShiftChunk - 1) / kShiftChunk));
       launch_l<ShiftCycle>(l, gs, dim3(kBlock), st, (const int32_t *)m.active.p,
                            (const int32_t *)m.mr_active.p, m.vv, m.sc);
     }
     {
-      KScope ks_("mr_seed", st);
+      KScope ks_("mr_seed", st, 2);
       launch_l<MrSeed>(l, gv, dim3(kBlock), st, (const int32_t *)m.mr_active.p, m.vv, m.sc);
     }
     m.check(st, iter, threshold, m.mr_active.p, m.active.p);

@@ -17,7 +17,7 @@ struct Rec {
 };
 
 std::mutex g_mu;
-bool g_enabled = false;
+int g_level = 0;
 std::vector<Rec> g_recs;       // events of the current measurement
 std::vector<hipEvent_t> g_pool;  // recycled events
 std::map<std::string, std::pair<double, long long>> g_done;  // resolved totals
@@ -51,10 +51,10 @@ void resolve_locked() {
 
 }  // namespace
 
-bool ktimer_enabled() { return g_enabled; }
+int ktimer_level() { return g_level; }
 
-KScope::KScope(const char *name, hipStream_t st) : st_(st) {
-  if (!g_enabled) return;
+KScope::KScope(const char *name, hipStream_t st, int level) : st_(st) {
+  if (level > g_level) return;
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_recs.size() > 20000) resolve_locked();  // bound the number of live events
   Rec r{name, take_event(), take_event()};
@@ -73,13 +73,14 @@ KScope::~KScope() {
 
 extern "C" {
 
-// Enable (1) or disable (0) kernel timing; both reset the accumulated totals.
+// Set the timing level (0 off, 1 operator applications and solver phases,
+// 2 every kernel launch); any call resets the accumulated totals.
 int sgw_kernel_timing(int enable) {
   try {
     std::lock_guard<std::mutex> lk(sgw::g_mu);
     sgw::resolve_locked();
     sgw::g_done.clear();
-    sgw::g_enabled = enable != 0;
+    sgw::g_level = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
     return 0;
   } catch (const sgw::Error &e) {
     sgw::set_error(e.msg);

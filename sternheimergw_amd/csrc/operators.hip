@@ -319,32 +319,42 @@ __device__ inline bool tile_active(const int32_t *__restrict__ active,
   return any != 0;
 }
 
-// becp[g][vt][p][vl] = D_p sum over the chunk partials (once per tile)
-__global__ __launch_bounds__(256) void k_pw_becp_reduce(PwDev d,
-                                                        const int32_t *__restrict__ active,
-                                                        const int32_t *__restrict__ gvec,
-                                                        const int32_t *__restrict__ gcnt,
-                                                        int maxper, int tiles, int nchunk,
-                                                        const double2 *__restrict__ part,
-                                                        double2 *__restrict__ becp) {
-  const int g = blockIdx.x, vt = blockIdx.y;
+// becp[g][vt][p][vl] = D_p sum over the chunk partials (once per tile); one
+// lane per output (p, vl), 64 outputs per block
+__global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
+                                                       const int32_t *__restrict__ active,
+                                                       const int32_t *__restrict__ gvec,
+                                                       const int32_t *__restrict__ gcnt,
+                                                       int maxper, int tiles, int nchunk,
+                                                       const double2 *__restrict__ part,
+                                                       double2 *__restrict__ becp) {
+  const int g = blockIdx.y, vt = blockIdx.z;
   const int cnt = gcnt[g], v0 = vt * kVT;
   if (v0 >= cnt) return;
   const int nv = min(kVT, cnt - v0), nproj = d.nproj;
+  const int o = blockIdx.x * 64 + threadIdx.x;
+  if (o >= nproj * kVT) return;
   if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   const int nck = (d.npw[g] + kChunk - 1) / kChunk;
-  const double2 *pp = part + ((int64_t)g * tiles + vt) * nchunk * nproj * kVT;
-  double2 *out = becp + ((int64_t)g * tiles + vt) * nproj * kVT;
-  for (int o = threadIdx.x; o < nproj * kVT; o += 256) {
-    double2 acc = mk(0, 0);
-    for (int c = 0; c < nck; ++c) {
-      const double2 t = pp[(int64_t)c * nproj * kVT + o];
-      acc.x += t.x;
-      acc.y += t.y;
-    }
-    const double dc = d.dcoef[(int64_t)g * nproj + o / kVT];
-    out[o] = mk(acc.x * dc, acc.y * dc);
+  const int64_t stride = (int64_t)nproj * kVT;
+  const double2 *pp = part + ((int64_t)g * tiles + vt) * nchunk * stride + o;
+  double2 a0 = mk(0, 0), a1 = mk(0, 0), a2 = mk(0, 0), a3 = mk(0, 0);
+  int c = 0;
+  for (; c + 4 <= nck; c += 4) {  // four independent loads in flight
+    const double2 t0 = pp[(int64_t)c * stride], t1 = pp[(int64_t)(c + 1) * stride];
+    const double2 t2 = pp[(int64_t)(c + 2) * stride], t3 = pp[(int64_t)(c + 3) * stride];
+    a0.x += t0.x; a0.y += t0.y;
+    a1.x += t1.x; a1.y += t1.y;
+    a2.x += t2.x; a2.y += t2.y;
+    a3.x += t3.x; a3.y += t3.y;
   }
+  for (; c < nck; ++c) {
+    const double2 t = pp[(int64_t)c * stride];
+    a0.x += t.x; a0.y += t.y;
+  }
+  const double sx = (a0.x + a1.x) + (a2.x + a3.x), sy = (a0.y + a1.y) + (a2.y + a3.y);
+  const double dc = d.dcoef[(int64_t)g * nproj + o / kVT];
+  becp[((int64_t)g * tiles + vt) * stride + o] = mk(sx * dc, sy * dc);
 }
 
 // part[((g*tiles + vt)*nchunk + chunk)][p][vl] = sum_{i in chunk} conj(proj_p(i)) x_v(i)
@@ -468,78 +478,6 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   }
 }
 
-__device__ inline double2 wave_sum(double2 v) {
-  for (int off = 32; off > 0; off >>= 1) {
-    v.x += __shfl_xor(v.x, off, 64);
-    v.y += __shfl_xor(v.y, off, 64);
-  }
-  return v;
-}
-
-// One block per vector: z forward transform, kinetic, projectors, shift.
-__global__ __launch_bounds__(kBlock) void k_pw_assemble(PwDev d, const double2 *__restrict__ in,
-                                                        int64_t sin, double2 *__restrict__ out,
-                                                        int64_t sout,
-                                                        const double2 *__restrict__ sigma,
-                                                        const int32_t *__restrict__ group,
-                                                        const int32_t *__restrict__ active,
-                                                        const double2 *__restrict__ S) {
-  extern __shared__ double2 smem[];
-  const int v = blockIdx.x;
-  if (active && !active[v]) return;
-  const int g = group ? group[v] : 0;
-  const int n3 = d.n3, nproj = d.nproj;
-  const int64_t ld = d.ld;
-  double2 *w3 = smem;           // n3
-  double2 *becp = w3 + n3;      // nproj
-  for (int i = threadIdx.x; i < n3; i += kBlock) w3[i] = d.tw3[i];
-  const double2 *x = in + v * sin;
-  const int npw = d.npw[g];
-  const double2 *pg = d.proj + (int64_t)g * nproj * ld;
-  // becp_p = <p|x>, one wave per projector
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int p = wid; p < nproj; p += kBlock / 64) {
-    const double2 *pp = pg + (int64_t)p * ld;
-    double2 acc = mk(0, 0);
-    for (int i = lane; i < npw; i += 64) cmacc(acc, pp[i], x[i]);
-    acc = wave_sum(acc);
-    if (lane == 0) {
-      const double dc = d.dcoef[(int64_t)g * nproj + p];
-      becp[p] = mk(acc.x * dc, acc.y * dc);
-    }
-  }
-  __syncthreads();
-  const double2 sg = sigma[v];
-  const double2 *Sv = S + (int64_t)v * d.nslot * n3;
-  const int32_t *slot = d.slot_of + (int64_t)g * ld, *m3 = d.m3_of + (int64_t)g * ld;
-  const double *kin = d.kin + (int64_t)g * ld;
-  double2 *o = out + v * sout;
-  for (int64_t i = threadIdx.x; i < ld; i += kBlock) {
-    if (i >= npw) {
-      o[i] = mk(0, 0);
-      continue;
-    }
-    const double2 xi = x[i];
-    // kinetic
-    double2 acc = mk(kin[i] * xi.x, kin[i] * xi.y);
-    // local: z forward transform of the column, 1/N
-    const double2 *col = Sv + (int64_t)slot[i] * n3;
-    const int mz = m3[i];
-    double2 loc = mk(0, 0);
-    for (int z = 0; z < n3; ++z) {
-      int e = (mz * z) % n3;
-      cmac(loc, col[z], cconj(w3[e]));
-    }
-    acc.x = fma(loc.x, d.inv_nbox, acc.x);
-    acc.y = fma(loc.y, d.inv_nbox, acc.y);
-    // projectors
-    for (int p = 0; p < nproj; ++p) cmac(acc, pg[(int64_t)p * ld + i], becp[p]);
-    // shift
-    cmac(acc, sg, xi);
-    o[i] = acc;
-  }
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -558,7 +496,7 @@ void DenseOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t i
                           const int32_t *) {
   dim3 grid((unsigned)((ld + kBlock - 1) / kBlock), nvec);
   {
-    KScope ks("dense_matvec", st);
+    KScope ks("happly", st);
     k_dense<<<grid, kBlock, 0, st>>>(n, a_.p, in, in_stride, out, out_stride, ld, d_sigma,
                                      d_active);
   }
@@ -695,6 +633,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
                        double2 *out, int64_t out_stride, const double2 *d_sigma, const double2 *,
                        const int32_t *d_active, const int32_t *, const int32_t *d_group) {
   if (nvec <= 0) return;
+  KScope whole("happly", st);
   SGW_REQUIRE(d_group != nullptr || ngroup_ == 1, SGW_E_ARG,
               "pw operator: a group index per vector is required with several groups");
   ensure_scratch(nvec);
@@ -716,15 +655,15 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   auto launch_fft = [&](auto tag) {
     constexpr int NB = decltype(tag)::value;
     {
-      KScope ks("pw_zinv", st);
+      KScope ks("pw_zinv", st, 2);
       k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T_.p);
     }
     {
-      KScope ks("pw_plane", st);
+      KScope ks("pw_plane", st, 2);
       k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, T_.p, d_group, d_active, S_.p);
     }
     {
-      KScope ks("pw_zfwd", st);
+      KScope ks("pw_zfwd", st, 2);
       k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p);
     }
     locv = loc_.p;
@@ -743,7 +682,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     size_t lds_plane =
         sizeof(double2) * (n1_ + n2_ + nslot_ + (size_t)nc1_ * n2_ + (size_t)n2_ * n1_);
     SGW_REQUIRE(lds_plane <= 64 * 1024, SGW_E_ARG, "pw operator: FFT plane does not fit in LDS");
-    KScope ks("pw_plane", st);
+    KScope ks("pw_plane", st, 2);
     k_pw_plane<<<dim3(n3_, nvec), kBlock, lds_plane, st>>>(d, in, in_stride, d_group, d_active,
                                                           S_.p);
   }
@@ -752,17 +691,18 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
               "pw operator: prepare() was not called for this batch of vectors");
   const int nchunk = (int)((ld + kChunk - 1) / kChunk);
   if (nproj_ > 0) {
-    KScope ks("pw_becp", st);
+    KScope ks("pw_becp", st, 2);
     k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
         d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
         part_.p);
-    k_pw_becp_reduce<<<dim3(ngroup_, plan_tiles_), 256, 0, st>>>(
+    k_pw_becp_reduce<<<dim3((unsigned)((nproj_ * kVT + 63) / 64), ngroup_, plan_tiles_), 64, 0,
+                       st>>>(
         d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk, part_.p, becp_.p);
   }
   SGW_HIP(hipGetLastError());
   {
     const size_t lds_asm = sizeof(double2) * ((size_t)nproj_ * kVT + n3_);
-    KScope ks("pw_assemble", st);
+    KScope ks("pw_assemble", st, 2);
     k_pw_assemble2<<<dim3(nchunk, ngroup_, plan_tiles_), 256, lds_asm, st>>>(
         d, in, in_stride, out, out_stride, d_sigma, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
         plan_tiles_, nchunk, becp_.p, S_.p, locv);

@@ -31,8 +31,10 @@ END MODULE fixture_problem
 
 PROGRAM sgw_fixture_test
 
-  USE, INTRINSIC :: iso_c_binding, ONLY: c_int32_t
+  USE, INTRINSIC :: iso_c_binding, ONLY: c_int32_t, c_int64_t, c_ptr, c_associated
   USE sgw_kinds,            ONLY: dp
+  USE sgw_capi,             ONLY: sgw_batch_config, sgw_operator_dense, sgw_operator_destroy, &
+                                  sgw_sternheimer_solve_batched
   USE fixture_problem,      ONLY: AA, linear_problem_apply
   USE bicgstab_module,      ONLY: bicgstab, bicgstab_type
   USE linear_solver_module, ONLY: linear_solver, linear_solver_config
@@ -135,6 +137,42 @@ PROGRAM sgw_fixture_test
   END DO
   IF (worst > 10.0_dp * threshold) nfail = nfail + 1
   WRITE(*, '(a,es10.3,a,i0)') 'select_solver [1,3]:   max residual ', worst, ' failures ', nfail
+
+  ! the batched entry on a dense device operator: two systems (bb and i*bb),
+  ! all shifts each, in the layout a green_function / solve_linter shim passes
+  BLOCK
+    TYPE(c_ptr) :: op
+    TYPE(sgw_batch_config) :: bcfg
+    INTEGER(c_int32_t) :: grp(2), berr(2), bit(2), st
+    COMPLEX(dp), ALLOCATABLE :: rhs2(:,:), sig2(:,:), x2(:,:,:)
+    INTEGER :: isys
+    op = sgw_operator_dense(AA, INT(vec_size, c_int64_t))
+    IF (.NOT. c_associated(op)) THEN
+      nfail = nfail + 1
+    ELSE
+      ALLOCATE(rhs2(vec_size, 2), sig2(num_shift, 2), x2(vec_size, num_shift, 2))
+      rhs2(:, 1) = bb
+      rhs2(:, 2) = (0.0_dp, 1.0_dp) * bb
+      sig2(:, 1) = sigma
+      sig2(:, 2) = sigma
+      grp = 0
+      bcfg = sgw_batch_config(2, num_shift, INT(vec_size, c_int64_t), 4, 10000, threshold)
+      st = sgw_sternheimer_solve_batched(op, bcfg, grp, rhs2, sig2, x2, berr, bit)
+      IF (st /= 0 .OR. ANY(berr /= 0)) nfail = nfail + 1
+      worst = 0
+      DO isys = 1, 2
+        DO ishift = 1, num_shift
+          CALL linear_problem_apply(sig2(ishift, isys), x2(:, ishift, isys), work)
+          work = work - rhs2(:, isys)
+          worst = MAX(worst, SQRT(SUM(ABS(work)**2)))
+        END DO
+      END DO
+      IF (worst > 10.0_dp * threshold) nfail = nfail + 1
+      IF (ANY(ABS(x2(:, :, 2) - (0.0_dp, 1.0_dp) * x2(:, :, 1)) > 1e-12_dp)) nfail = nfail + 1
+      CALL sgw_operator_destroy(op)
+    END IF
+    WRITE(*, '(a,es10.3,a,i0)') 'batched (dense op):    max residual ', worst, ' failures ', nfail
+  END BLOCK
 
   IF (nfail /= 0) THEN
     WRITE(*, '(a)') 'FAIL'

@@ -43,6 +43,30 @@ MODULE sgw_capi
     REAL(c_double)     :: abs_threshold
   END TYPE sgw_linear_solver_config
 
+  !> sgw_pw_desc: the plane-wave operator's data (include/sgw.h)
+  TYPE, BIND(C) :: sgw_pw_desc
+    INTEGER(c_int32_t) :: n1, n2, n3
+    TYPE(c_ptr)        :: vloc      ! REAL(dp) vloc(n1*n2*n3)
+    INTEGER(c_int32_t) :: ngroup
+    TYPE(c_ptr)        :: npw       ! INTEGER(c_int32_t) npw(ngroup)
+    INTEGER(c_int64_t) :: ld
+    TYPE(c_ptr)        :: mill      ! INTEGER(c_int32_t) mill(3, ld, ngroup)
+    TYPE(c_ptr)        :: kin       ! REAL(dp) kin(ld, ngroup), Ry
+    INTEGER(c_int32_t) :: nproj
+    TYPE(c_ptr)        :: proj      ! COMPLEX(dp) proj(ld, nproj, ngroup)
+    TYPE(c_ptr)        :: dcoef     ! REAL(dp) dcoef(nproj, ngroup), Ry
+  END TYPE sgw_pw_desc
+
+  !> sgw_batch_config
+  TYPE, BIND(C) :: sgw_batch_config
+    INTEGER(c_int32_t) :: nsys
+    INTEGER(c_int32_t) :: nshift
+    INTEGER(c_int64_t) :: ld
+    INTEGER(c_int32_t) :: lmax
+    INTEGER(c_int32_t) :: max_iter
+    REAL(c_double)     :: threshold
+  END TYPE sgw_batch_config
+
   INTERFACE
 
     FUNCTION sgw_select_solver(config, AA, ctx, bb, n, sigma, nshift, xx) &
@@ -94,6 +118,42 @@ MODULE sgw_capi
       IMPORT :: c_ptr
       TYPE(c_ptr) :: msg
     END FUNCTION sgw_last_error
+
+    !> Dense device operator (the unit test's linear_problem_apply)
+    FUNCTION sgw_operator_dense(a, n) BIND(C, name="sgw_operator_dense") RESULT(op)
+      IMPORT :: c_ptr, c_int64_t, c_double_complex
+      INTEGER(c_int64_t), VALUE :: n
+      COMPLEX(c_double_complex), INTENT(IN) :: a(n, n)
+      TYPE(c_ptr) :: op
+    END FUNCTION sgw_operator_dense
+
+    !> Plane-wave device operator (h_psi + alpha_pv P_v of linear_op)
+    FUNCTION sgw_operator_pw(desc) BIND(C, name="sgw_operator_pw") RESULT(op)
+      IMPORT :: c_ptr, sgw_pw_desc
+      TYPE(sgw_pw_desc), INTENT(IN) :: desc
+      TYPE(c_ptr) :: op
+    END FUNCTION sgw_operator_pw
+
+    SUBROUTINE sgw_operator_destroy(op) BIND(C, name="sgw_operator_destroy")
+      IMPORT :: c_ptr
+      TYPE(c_ptr), VALUE :: op
+    END SUBROUTINE sgw_operator_destroy
+
+    !> All (system, shift) pairs of a band/q or G' loop in one call;
+    !! rhs(ld, nsys), sigma(nshift, nsys), x(ld, nshift, nsys) in Fortran order
+    FUNCTION sgw_sternheimer_solve_batched(op, cfg, group, rhs, sigma, x, ierr, iters) &
+        BIND(C, name="sgw_sternheimer_solve_batched") RESULT(status)
+      IMPORT :: c_ptr, c_int32_t, c_double_complex, sgw_batch_config
+      TYPE(c_ptr), VALUE :: op
+      TYPE(sgw_batch_config), INTENT(IN) :: cfg
+      INTEGER(c_int32_t), INTENT(IN) :: group(*)
+      COMPLEX(c_double_complex), INTENT(IN) :: rhs(*)
+      COMPLEX(c_double_complex), INTENT(IN) :: sigma(*)
+      COMPLEX(c_double_complex), INTENT(OUT) :: x(*)
+      INTEGER(c_int32_t), INTENT(OUT) :: ierr(*)
+      INTEGER(c_int32_t), INTENT(OUT) :: iters(*)
+      INTEGER(c_int32_t) :: status
+    END FUNCTION sgw_sternheimer_solve_batched
 
   END INTERFACE
 

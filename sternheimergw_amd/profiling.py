@@ -6,6 +6,10 @@
         per-kernel average counter value (e.g. FETCH_SIZE or WRITE_SIZE, KiB
         per dispatch); with copy-bytes, the largest copy kernel's reading is
         used to scale KiB into bytes (calibration against a copy of known size)
+    python -m sternheimergw_amd.profiling happly <trace.db> <fetch.db> <write.db> <copy-bytes>
+        the plane-wave H-apply as one unit (all k_pw_* kernels of one
+        application): average duration per application from the kernel trace
+        and calibrated HBM bytes read and written per application, as JSON
 """
 from __future__ import annotations
 
@@ -69,11 +73,50 @@ def pmc(db: str, copy_bytes: int = 0) -> str:
     return "\n".join(out)
 
 
+def _pmc_per_kernel(db: str, copy_bytes: int):
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    agg = collections.defaultdict(list)
+    for name, _c, val in rows:
+        agg[name].append(val)
+    copies = [max(v) for n, v in agg.items() if "copy" in n.lower() or "elementwise" in n.lower()]
+    scale = copy_bytes / (max(copies) * 1024.0) if copies and max(copies) else None
+    return agg, scale
+
+
+def happly(trace_db: str, fetch_db: str, write_db: str, copy_bytes: int) -> dict:
+    """The H-apply's kernels launch once each per application, so the
+    application's average is the sum of the per-kernel averages."""
+    con = sqlite3.connect(trace_db)
+    rows = con.execute("select name, duration from kernels").fetchall()
+    dur = collections.defaultdict(list)
+    for name, d in rows:
+        if "k_pw_" in name:
+            dur[_short(name)].append(d)
+    launches = {k: len(v) for k, v in dur.items()}
+    avg_ns = {k: sum(v) / len(v) for k, v in dur.items()}
+    out = {"kernels": sorted(dur), "launches": launches,
+           "avg_us": {k: round(v / 1e3, 3) for k, v in avg_ns.items()},
+           "happly_avg_ms": round(sum(avg_ns.values()) / 1e6, 5)}
+    for tag, db in (("fetch", fetch_db), ("write", write_db)):
+        agg, scale = _pmc_per_kernel(db, copy_bytes)
+        per = {_short(n): sum(v) / len(v) * 1024.0 * (scale or 1.0)
+               for n, v in agg.items() if "k_pw_" in n}
+        out[f"{tag}_factor"] = round(scale, 4) if scale else None
+        out[f"{tag}_bytes_per_happly"] = round(sum(per.values()))
+        out[f"{tag}_bytes_per_kernel"] = {k: round(v) for k, v in per.items()}
+    out["traffic_bytes_per_happly"] = out["fetch_bytes_per_happly"] + out["write_bytes_per_happly"]
+    return out
+
+
 def main(argv):
-    if len(argv) < 3 or argv[1] not in ("trace", "pmc"):
+    if len(argv) < 3 or argv[1] not in ("trace", "pmc", "happly"):
         print(__doc__)
         return 2
-    if argv[1] == "trace":
+    if argv[1] == "happly":
+        import json
+        print(json.dumps(happly(argv[2], argv[3], argv[4], int(argv[5])), indent=1))
+    elif argv[1] == "trace":
         print(trace(argv[2]))
     else:
         print(pmc(argv[2], int(argv[3]) if len(argv) > 3 else 0))
