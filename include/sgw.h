@@ -167,6 +167,11 @@ typedef struct {
   int32_t lmax;        /* BiCGStab(l) degree, 1..15                   */
   int32_t max_iter;
   double threshold;    /* absolute, on the seed residual              */
+  int32_t shift_test;  /* 0: the reference's test on the seed residual
+                          only; 1: also every shifted residual, through
+                          |1/(theta phi)| ||r_seed|| (no counterpart in
+                          the reference, which can leave a shift above
+                          the threshold)                               */
 } sgw_batch_config;
 
 typedef struct sgw_batch_solver sgw_batch_solver;
@@ -189,6 +194,26 @@ int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
                                   const sgw_complex *sigma, sgw_complex *x, int32_t *ierr,
                                   int32_t *iters);
 
+/* Adjoint of the batched solve (no counterpart in the reference, which has no
+ * derivatives).  For the objective
+ *   L = sum_{s,k} weight[s][k] Re <c[s] | x[s][k]>,
+ * with x the stored solutions of a forward sgw_batch_solver_run of `solver`
+ * (rhs b, shifts sigma), it solves the backward systems
+ *   (A_{group[s]} + conj(sigma[s][k])) y[s][k] = c[s]
+ * with the same solver and configuration, and, for a plane-wave operator,
+ * writes the gradient of L with respect to V_loc(r) on the FFT box:
+ *   d_grad[i1 + n1*(i2 + n2*i3)] = -(1/N) sum_{s,k} weight[s][k] Re[conj(y(r)) x(r)],
+ * x(r) = invfft(x(G)) (QE sign, no factor).  Device arrays: d_c[nsys][ld],
+ * d_sigma[nsys][nshift] (h_sigma its host copy), d_weight[nsys][nshift] (real),
+ * d_x and d_y [nsys][nshift][ld]; d_grad may be NULL (no gradient).
+ * h_objective (nullable) receives L; h_ierr (nullable) the backward ierr
+ * per system.  Returns the largest backward ierr or an API error. */
+int sgw_sternheimer_adjoint(sgw_batch_solver *solver, const int32_t *d_group,
+                            const sgw_complex *d_c, const sgw_complex *d_sigma,
+                            const sgw_complex *h_sigma, const double *d_weight,
+                            const sgw_complex *d_x, sgw_complex *d_y, double *d_grad,
+                            double *h_objective, int32_t *h_ierr, void *stream);
+
 /* True relative residuals ||rhs[s] - (A + sigma[s][k]) x[s][k]|| / ||rhs[s]||
  * recomputed in FP64 on the device for every (system, shift); device arrays,
  * result d_res[nsys][nshift]. */
@@ -196,6 +221,46 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
                               const int32_t *d_group, const sgw_complex *d_rhs,
                               const sgw_complex *d_sigma, const sgw_complex *d_x,
                               double *d_res, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Sigma_c frequency convolution (phys/corr/src/sigma.f90:705-745, the
+ * accumulation sigma(:,:,isigma) += alpha * w * [G (x) W] over the Green's
+ * function frequencies), restated as a batched complex GEMM on the device:
+ *   Sigma[i] (+)= alpha * sum_j weight[j] * G[i][j] * W[j],
+ * i < nout, j < nin, every matrix n x n complex, column-major, leading
+ * dimension ld; G[i][j] at d_g + i*g_stride_i + j*g_stride_j, W[j] at
+ * d_w + j*w_stride, Sigma[i] at d_sigma + i*s_stride (element offsets).
+ * The reference forms this product in real space (sigma_prod,
+ * sigma.f90:417-523); here it is the G'' contraction in reciprocal space.
+ * ------------------------------------------------------------------------- */
+#define SGW_SIGMA_FP64 0   /* FP64 operands, FP64 MFMA, FP64 accumulation */
+
+typedef struct {
+  int32_t n;
+  int64_t ld;
+  int32_t nout, nin;
+  int64_t g_stride_i, g_stride_j, w_stride, s_stride;
+  int32_t precision;   /* SGW_SIGMA_*                                    */
+  int32_t accumulate;  /* 1: Sigma += ..., 0: Sigma = ...                */
+} sgw_sigma_conv_desc;
+
+/* Device arrays; d_weight[nin] (real) on the device.  Asynchronous on `stream`. */
+int sgw_sigma_conv_device(const sgw_sigma_conv_desc *desc, const sgw_complex *d_g,
+                          const sgw_complex *d_w, const double *d_weight, sgw_complex alpha,
+                          sgw_complex *d_sigma, void *stream);
+
+/* Seeded synthetic inputs of the convolution benchmark, generated on the
+ * device (the full G stack at n = 1500, 64 x 64 frequencies is 147 GB):
+ * G[i][j], i0 <= i < i0 + ni (written at d_g + (i - i0)*g_stride_i + ...),
+ * from a counter-based hash scaled by 1 / ((1 + nu[i*nin + j]) sqrt(n));
+ * W[j](a, b) = cw[j] exp(-|g_a - g_b|^2 / (2 length^2)) e^{i(theta_a - theta_b)}
+ * / ((1 + |g_a|^2)(1 + |g_b|^2)), Hermitian positive semidefinite.
+ * Either output may be NULL. */
+int sgw_sigma_synthetic_device(int32_t n, int64_t ld, int32_t i0, int32_t ni, int32_t nout,
+                               int32_t nin, uint64_t seed, const double *d_nu,
+                               sgw_complex *d_g, int64_t g_stride_i, int64_t g_stride_j,
+                               const double *d_gvec, const double *d_theta, const double *d_cw,
+                               double length, sgw_complex *d_w, int64_t w_stride, void *stream);
 
 /* Kernel timing with HIP events recorded on each kernel's launch stream.
  * sgw_kernel_timing(level) sets the level and resets the totals: 0 off,

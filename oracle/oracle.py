@@ -43,6 +43,8 @@ def lib():
         L.oracle_gram_schmidt.argtypes = [I32, V, I64, I32, V]
         L.oracle_bicgstab.restype = I32
         L.oracle_bicgstab.argtypes = [I32, D, I32, V, V, V, I64, V, I32, V, V]
+        L.oracle_bicgstab_ex.restype = I32
+        L.oracle_bicgstab_ex.argtypes = [I32, D, I32, V, V, V, I64, V, I32, V, V, I32]
         L.oracle_linear_solver.restype = I32
         L.oracle_linear_solver.argtypes = [I32, D, V, V, V, I64, V, I32, V, V]
         L.oracle_select_solver.restype = I32
@@ -103,14 +105,16 @@ class PwOp:
         return ax
 
 
-def bicgstab(op, b, sigma, lmax=4, threshold=1e-4, max_iter=10000):
-    """Returns x (nshift, n), ierr, iterations."""
+def bicgstab(op, b, sigma, lmax=4, threshold=1e-4, max_iter=10000, shift_test=False):
+    """Returns x (nshift, n), ierr, iterations.  shift_test=True also waits
+    for every shifted residual estimate (an extension of the reference's test)."""
     b = np.ascontiguousarray(b, dtype=np.complex128)
     sigma = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.complex128)
     x = np.zeros((sigma.size, b.size), dtype=np.complex128)
     it = ctypes.c_int(0)
-    ierr = lib().oracle_bicgstab(lmax, threshold, max_iter, op.fn, op.ctx_ptr, _p(b), b.size,
-                                 _p(sigma), sigma.size, _p(x), ctypes.byref(it))
+    ierr = lib().oracle_bicgstab_ex(lmax, threshold, max_iter, op.fn, op.ctx_ptr, _p(b), b.size,
+                                    _p(sigma), sigma.size, _p(x), ctypes.byref(it),
+                                    int(shift_test))
     return x, ierr, it.value
 
 

@@ -45,6 +45,13 @@ int oracle_bicgstab(int lmax, double threshold, int max_iter,
                     oracle_op_fn op, void *ctx, const ozc *b, int64_t n,
                     const ozc *sigma, int nshift, ozc *x, int *iters_out);
 
+/* oracle_bicgstab with shift_test = 1: a solve also waits for every shifted
+ * residual estimate |1/(theta phi)| ||r_seed|| to pass (an extension; 0 is
+ * the reference's test). */
+int oracle_bicgstab_ex(int lmax, double threshold, int max_iter,
+                       oracle_op_fn op, void *ctx, const ozc *b, int64_t n,
+                       const ozc *sigma, int nshift, ozc *x, int *iters_out, int shift_test);
+
 int oracle_linear_solver(int max_iter, double threshold,
                          oracle_op_fn op, void *ctx, const ozc *b, int64_t n,
                          const ozc *sigma, int nshift, ozc *x, int *basis_out);

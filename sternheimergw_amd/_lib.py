@@ -59,8 +59,20 @@ class sgw_pw_desc(ctypes.Structure):
 
 class sgw_batch_config(ctypes.Structure):
     _fields_ = [("nsys", c_int32), ("nshift", c_int32), ("ld", c_int64), ("lmax", c_int32),
-                ("max_iter", c_int32), ("threshold", c_double)]
+                ("max_iter", c_int32), ("threshold", c_double), ("shift_test", c_int32)]
 
+
+class sgw_complex(ctypes.Structure):
+    _fields_ = [("re", c_double), ("im", c_double)]
+
+
+class sgw_sigma_conv_desc(ctypes.Structure):
+    _fields_ = [("n", c_int32), ("ld", c_int64), ("nout", c_int32), ("nin", c_int32),
+                ("g_stride_i", c_int64), ("g_stride_j", c_int64), ("w_stride", c_int64),
+                ("s_stride", c_int64), ("precision", c_int32), ("accumulate", c_int32)]
+
+
+SIGMA_FP64 = 0
 
 # (name, restype, argtypes) of every exported entry point of include/sgw.h
 SIGNATURES = [
@@ -89,6 +101,15 @@ SIGNATURES = [
                                                 c_void_p]),
     ("sgw_residual_norms_device", c_int32, [c_void_p, c_int32, c_int32, c_int64, c_void_p,
                                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    ("sgw_sternheimer_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p]),
+    ("sgw_sigma_conv_device", c_int32, [ctypes.POINTER(sgw_sigma_conv_desc), c_void_p, c_void_p,
+                                        c_void_p, sgw_complex, c_void_p, c_void_p]),
+    ("sgw_sigma_synthetic_device", c_int32, [c_int32, c_int64, c_int32, c_int32, c_int32, c_int32,
+                                             ctypes.c_uint64, c_void_p, c_void_p, c_int64, c_int64,
+                                             c_void_p, c_void_p, c_void_p, c_double, c_void_p,
+                                             c_int64, c_void_p]),
     ("sgw_kernel_timing", c_int32, [c_int32]),
     ("sgw_kernel_timing_query", c_int32, [ctypes.c_char_p, ctypes.POINTER(c_double),
                                           ctypes.POINTER(c_int64)]),

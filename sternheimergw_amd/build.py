@@ -22,7 +22,8 @@ LIB = os.path.join(HERE, "libsgw_hip.so")
 FORTRAN = os.path.join(HERE, "fortran")
 OBJDIR = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["operators.hip", "bicgstab.hip", "subspace.hip", "ktimer.hip", "capi.hip"]
+HIP_SOURCES = ["operators.hip", "bicgstab.hip", "subspace.hip", "sigma.hip", "adjoint.hip",
+               "ktimer.hip", "capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

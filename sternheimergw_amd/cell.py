@@ -194,6 +194,8 @@ class Workload:
     sigma: np.ndarray                # (nsys, nshift) complex shifts -(eps_v + omega)
     nbnd: int
     dense_h: Optional[List[np.ndarray]] = None  # per q dense H (small cells only)
+    npert: int = 1                   # plane-wave perturbations e^{i(q+G)r} per (q, band)
+    pert_q2: Optional[np.ndarray] = None  # (nsys,) |q+G|^2 (bohr^-2) of each system's perturbation
 
     @property
     def nsys(self):
@@ -235,7 +237,10 @@ def gauss_legendre(x1: float, x2: float, n: int):
 def build_workload(name: str, *, box, ecut_ha: float, nbnd: int, lchan: List[int],
                    qpoints: np.ndarray, omega_ha: np.ndarray, ecut_scale=None,
                    alat: float = 10.26, vloc_rms_ha: float = 0.3, seed: int = SEED_BASE,
-                   keep_dense: bool = False, normalize_rhs: bool = True) -> Workload:
+                   keep_dense: bool = False, normalize_rhs: bool = True,
+                   pert_mill: Optional[np.ndarray] = None) -> Workload:
+    """pert_mill (npert, 3): Miller indices G of the plane-wave perturbations
+    e^{i(q+G)r}; system s = (iq * nbnd + v) * npert + ip.  Default: G = 0."""
     rng = np.random.Generator(np.random.PCG64(seed))
     a, b, omega = fcc_lattice(alat)
     tau = np.array([[0.0, 0.0, 0.0], [alat / 4, alat / 4, alat / 4]])
@@ -284,31 +289,40 @@ def build_workload(name: str, *, box, ecut_ha: float, nbnd: int, lchan: List[int
     # right-hand sides b = -P_c S_q psi_v (dvqpsi_us.f90:99-132 with a G=0 plane
     # wave perturbation, then QE orthogonalize = -P_c^+; solve_linter.f90:331,337)
     omega_ry = np.asarray(omega_ha, dtype=np.complex128) * HA2RY
-    rhs = np.zeros((nq * nbnd, ld), dtype=np.complex128)
-    group = np.zeros(nq * nbnd, dtype=np.int32)
-    sigma = np.zeros((nq * nbnd, omega_ry.size), dtype=np.complex128)
+    pm = np.zeros((1, 3), dtype=np.int64) if pert_mill is None else np.asarray(pert_mill)
+    npert = pm.shape[0]
+    nsys = nq * nbnd * npert
+    rhs = np.zeros((nsys, ld), dtype=np.complex128)
+    group = np.zeros(nsys, dtype=np.int32)
+    sigma = np.zeros((nsys, omega_ry.size), dtype=np.complex128)
+    pert_q2 = np.zeros(nsys)
     key_k = {tuple(m): i for i, m in enumerate(sk.mill)}
     for iq in range(nq):
         sq = spheres[iq]
-        idx = np.array([key_k.get(tuple(m), -1) for m in sq.mill])
-        hit = idx >= 0
-        for v in range(nbnd):
-            s = iq * nbnd + v
-            dv = np.zeros(sq.npw, dtype=np.complex128)
-            dv[hit] = psi_k[idx[hit], v]
-            ev = evqs[iq]
-            dv = dv - ev.T @ (ev.conj() @ dv)
-            dv = -dv
-            if normalize_rhs:
-                dv /= np.linalg.norm(dv)
-            rhs[s, : sq.npw] = dv
-            group[s] = iq
-            # solve_linter.f90:368-369: shifts -(et(v) + omega)
-            sigma[s] = -(eps_k[v] + omega_ry)
+        ev = evqs[iq]
+        for ip in range(npert):
+            # e^{i(q+G)r} psi_v: coefficient at k+q+G2 is psi_v(G2 - G)
+            idx = np.array([key_k.get(tuple(m - pm[ip]), -1) for m in sq.mill])
+            hit = idx >= 0
+            qg = qpoints[iq] + pm[ip] @ b
+            for v in range(nbnd):
+                s = (iq * nbnd + v) * npert + ip
+                dv = np.zeros(sq.npw, dtype=np.complex128)
+                dv[hit] = psi_k[idx[hit], v]
+                dv = dv - ev.T @ (ev.conj() @ dv)
+                dv = -dv
+                if normalize_rhs:
+                    dv /= np.linalg.norm(dv)
+                rhs[s, : sq.npw] = dv
+                group[s] = iq
+                pert_q2[s] = float(qg @ qg)
+                # solve_linter.f90:368-369: shifts -(et(v) + omega)
+                sigma[s] = -(eps_k[v] + omega_ry)
     return Workload(name=name, box=tuple(box), alat=alat, vloc=vloc, spheres=spheres, ld=ld,
                     nproj_beta=nper * len(tau), proj=proj, dcoef=dcoef, alpha_pv=alpha_pv,
                     eps_v=eps_k, evq=evqs, rhs=rhs, group=group, omega=omega_ry,
-                    sigma=sigma, nbnd=nbnd, dense_h=dense if keep_dense else None)
+                    sigma=sigma, nbnd=nbnd, dense_h=dense if keep_dense else None,
+                    npert=npert, pert_q2=pert_q2)
 
 
 def workload_c1(keep_dense=True, box=(24, 24, 24), **kw) -> Workload:
@@ -319,6 +333,36 @@ def workload_c1(keep_dense=True, box=(24, 24, 24), **kw) -> Workload:
     return build_workload("c1", box=box, ecut_ha=6.0, nbnd=4, lchan=[0, 1], qpoints=q,
                           omega_ha=1j * np.linspace(0.0, 2.0, 8), keep_dense=keep_dense,
                           seed=SEED_BASE + 1, **kw)
+
+
+def shortest_millers(n: int, alat: float = 10.26) -> np.ndarray:
+    """Miller indices of the n shortest reciprocal-lattice vectors of the fcc
+    cell, sorted by length, then by Miller index."""
+    _, b, _ = fcc_lattice(alat)
+    bmin = np.min(np.linalg.norm(b, axis=1))
+    m = 1
+    while True:
+        r = np.arange(-m, m + 1)
+        mill = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+        g2 = ((mill @ b) ** 2).sum(1)
+        order = np.lexsort((mill[:, 2], mill[:, 1], mill[:, 0], np.round(g2, 10)))
+        # the n-th shell must lie inside the sphere inscribed in the searched cube
+        if len(order) >= n and g2[order[n - 1]] < (m * bmin / math.sqrt(3.0)) ** 2:
+            return np.ascontiguousarray(mill[order[:n]])
+        m += 1
+
+
+def workload_c5(nq: int = 4, nbnd: int = 8, nshift: int = 16, ng: int = 500,
+                box=(48, 48, 48), **kw) -> Workload:
+    """Adjoint workload: the 6 Ha cell of the plain configuration on a 48^3
+    box, 8 bands x 4 seeded q x ng plane-wave perturbations e^{i(q+G)r}
+    (the ng shortest G) x nshift Gauss-Legendre imaginary shifts on [0, 4 Ha];
+    right-hand sides -P_c e^{i(q+G)r} psi_v unnormalised (chi0 columns)."""
+    q, _ = c2_qpoints(nq, spread=0.0, seed=SEED_BASE + 5)
+    w, _ = gauss_legendre(0.0, 4.0, nshift)
+    return build_workload("c5", box=box, ecut_ha=6.0, nbnd=nbnd, lchan=[0, 1], qpoints=q,
+                          omega_ha=1j * w, seed=SEED_BASE + 5, normalize_rhs=False,
+                          pert_mill=shortest_millers(ng), **kw)
 
 
 def c2_qpoints(nq: int = 16, spread: float = 0.08, seed: int = SEED_BASE + 2):

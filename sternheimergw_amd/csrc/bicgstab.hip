@@ -442,9 +442,14 @@ __global__ __launch_bounds__(NT) void k_mr_mgs(const int32_t *__restrict__ activ
 // systems in `in`: out[s] = in[s] and not converged.  After the BiCG half
 // `out` is mr_active (the MR half runs only for those); after the MR half
 // `in` is mr_active and `out` the active set of the next iteration.
+// shift_test = 0 is the reference's test on the seed residual alone; with
+// shift_test = 1 every shifted residual must pass too, through its estimate
+// ||r^sigma|| = |1 / (theta^sigma phi^sigma)| ||r_seed|| (the collinearity of
+// Frommer's recurrences, bicgstab.f90:637-643), which can exceed the seed's.
 __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restrict__ in,
                         int32_t *__restrict__ out, const double *__restrict__ norm,
-                        int32_t *__restrict__ iters, int32_t *__restrict__ state) {
+                        int32_t *__restrict__ iters, int32_t *__restrict__ state, Scal sc,
+                        int nsh, int shift_test) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
   if (!in[s]) {
@@ -452,6 +457,14 @@ __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restric
     return;
   }
   double nr = norm[s];
+  if (shift_test) {
+    double zmax = 1.0;
+    for (int k = 0; k < nsh; ++k) {
+      const double2 z = cmul(sc.inv_theta[s * nsh + k], sc.inv_phi[s * nsh + k]);
+      zmax = fmax(zmax, sqrt(z.x * z.x + z.y * z.y));
+    }
+    nr *= zmax;
+  }
   int keep = 1;
   if (nr < thr) {
     keep = 0;
@@ -643,11 +656,13 @@ struct BatchBicgstab::Impl {
 
   const int32_t *group_dev = nullptr;
 
+  int shift_test = 0;
+
   // ||r_0|| test on the systems of `in`; survivors go to `out`
   void check(hipStream_t st, int iter, double thr, int32_t *in, int32_t *out) {
     k_norm<<<nsys, kBlock, 0, st>>>(nsys, in, R.p, (l + 1) * ld, ld, norms.p);
     k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, in, out, norms.p, iters.p,
-                                               state.p);
+                                               state.p, sc, nsh, shift_test);
   }
 };
 
@@ -659,8 +674,9 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
                           const double2 *d_sigma, const double2 *h_sigma,
                           const int32_t *d_group, double threshold, int max_iter,
                           double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
-                          int check_every) {
+                          int check_every, int shift_test) {
   Impl &m = *impl_;
+  m.shift_test = shift_test;
   const int S = m.nsys, l = m.l;
   SGW_REQUIRE(d_group != nullptr || op.ngroups() == 1, SGW_E_ARG,
               "bicgstab: a group index per system is required with several groups");

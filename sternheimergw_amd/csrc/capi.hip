@@ -7,18 +7,9 @@
 #include <string>
 
 #include "bicgstab.h"
+#include "handles.h"
 #include "operators.h"
 #include "subspace.h"
-
-struct sgw_operator {
-  std::unique_ptr<sgw::Operator> op;
-};
-
-struct sgw_batch_solver {
-  sgw_operator *op;
-  sgw_batch_config cfg;
-  std::unique_ptr<sgw::BatchBicgstab> engine;
-};
 
 namespace sgw {
 
@@ -343,7 +334,7 @@ int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
     std::vector<int32_t> ierr(c.nsys), it(c.nsys);
     solver->engine->solve(*solver->op->op, (hipStream_t)stream, D2(d_rhs), c.ld, D2(d_sigma),
                           D2(h_sigma), d_group, c.threshold, c.max_iter, D2(d_x), c.ld,
-                          ierr.data(), it.data(), 1);
+                          ierr.data(), it.data(), 1, c.shift_test);
     int worst = 0;
     for (int s = 0; s < c.nsys; ++s) worst = std::max(worst, ierr[s]);
     if (h_ierr) std::memcpy(h_ierr, ierr.data(), sizeof(int32_t) * c.nsys);
@@ -377,7 +368,7 @@ int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
     BatchBicgstab eng(S, K, cfg->lmax, ld);
     std::vector<int32_t> e(S), it(S);
     eng.solve(*op->op, st, d_rhs.p, ld, d_sig.p, D2(sigma), d_g.p, cfg->threshold, cfg->max_iter,
-              d_x.p, ld, e.data(), it.data(), 1);
+              d_x.p, ld, e.data(), it.data(), 1, cfg->shift_test);
     SGW_HIP(hipMemcpy(x, d_x.p, sizeof(double2) * S * K * ld, hipMemcpyDeviceToHost));
     int worst = 0;
     for (int s = 0; s < S; ++s) worst = std::max(worst, e[s]);

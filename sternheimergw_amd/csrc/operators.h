@@ -58,6 +58,15 @@ class PwOperator : public Operator {
   int ngroups() const override { return ngroup_; }
   void prepare(const int32_t *h_group, int nvec) override;
 
+  // Adjoint gradient partials: G[b][z][x][y] (+)= sum over pairs p = b, b+nb, ...
+  // (p < np) of w[p] Re(conj(y_p(r)) x_p(r)); G has nb * box_size() doubles.
+  void grad_accumulate(hipStream_t st, int np, const double2 *x, const double2 *y,
+                       int64_t stride, const int32_t *d_group, const double *d_w, int nb,
+                       double *G, bool accumulate);
+  int box_n() const { return n1_; }
+  int64_t column_slots() const { return nslot_; }
+  int64_t box_size() const { return (int64_t)n1_ * n2_ * n3_; }
+
  private:
   void ensure_scratch(int nvec);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]

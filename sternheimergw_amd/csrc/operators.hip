@@ -300,6 +300,75 @@ __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const double2 *__r
   }
 }
 
+// Y then X inverse transforms of one z-plane of the sphere columns Tz (in
+// LDS as T) into the registers of lane y: buf[x] = value at (x, y, z).
+// P (nc1 x N, LDS) is the row buffer; lanes >= N return with buf untouched.
+template <int N>
+__device__ inline void plane_to_real(const PwDev &d, const double2 *__restrict__ Tz, double2 *T,
+                                     double2 *P, double2 (&buf)[N]) {
+  const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M1 = d.M1, M2 = d.M2;
+  const int lane = threadIdx.x;
+  __syncthreads();
+  for (int c = lane; c < nslot; c += 64) T[c] = Tz[c];
+  __syncthreads();
+  if (lane < nc1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      buf[i] = (m >= -M2 && m <= M2) ? T[lane * nc2 + m + M2] : mk(0, 0);
+    }
+    reg_fft<N, +1>(buf);
+#pragma unroll
+    for (int i = 0; i < N; ++i) P[lane * N + i] = buf[i];
+  }
+  __syncthreads();
+  if (lane < N) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      buf[i] = (m >= -M1 && m <= M1) ? P[(m + M1) * N + lane] : mk(0, 0);
+    }
+    reg_fft<N, +1>(buf);
+  }
+}
+
+// Adjoint gradient of one z-plane over a strided set of vector pairs:
+//   G[b][z][x][y] += sum_{p = b, b + nb, ...} w_p Re(conj(y_p(r)) x_p(r)),
+// from the z-transformed columns Tx (x_p) and Ty (y_p).  Block (z, b) owns
+// its plane of G exclusively; the x plane waits in LDS (Xs[x][y]) for y.
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_plane_grad(PwDev d, const double2 *__restrict__ Tx,
+                                                      const double2 *__restrict__ Ty, int np,
+                                                      const double *__restrict__ w,
+                                                      double *__restrict__ G, int accumulate) {
+  extern __shared__ double2 smem[];
+  const int z = blockIdx.x, b = blockIdx.y, nb = gridDim.y;
+  const int lane = threadIdx.x;
+  double2 *T = smem;               // nslot
+  double2 *P = T + d.nslot;        // nc1 * N
+  double2 *Xs = P + d.nc1 * N;     // N * N
+  double *Gz = G + ((int64_t)b * N + z) * N * N;
+  if (!accumulate && lane < N)
+    for (int x = 0; x < N; ++x) Gz[(int64_t)x * N + lane] = 0.0;
+  double2 buf[N];
+  for (int p = b; p < np; p += nb) {
+    plane_to_real<N>(d, Tx + ((int64_t)p * N + z) * d.nslot, T, P, buf);
+    if (lane < N)
+#pragma unroll
+      for (int x = 0; x < N; ++x) Xs[x * N + lane] = buf[x];
+    plane_to_real<N>(d, Ty + ((int64_t)p * N + z) * d.nslot, T, P, buf);
+    if (lane < N) {
+      const double wp = w[p];
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const double2 a = Xs[x * N + lane];
+        double *g = Gz + (int64_t)x * N + lane;
+        *g = fma(wp, buf[x].x * a.x + buf[x].y * a.y, *g);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Projector contraction per k+q group: the vectors of one group share the
 // projector block proj[g] (nproj x npw), so the two complex GEMMs
@@ -706,6 +775,47 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     k_pw_assemble2<<<dim3(nchunk, ngroup_, plan_tiles_), 256, lds_asm, st>>>(
         d, in, in_stride, out, out_stride, d_sigma, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
         plan_tiles_, nchunk, becp_.p, S_.p, locv);
+  }
+  SGW_HIP(hipGetLastError());
+}
+
+void PwOperator::grad_accumulate(hipStream_t st, int np, const double2 *x, const double2 *y,
+                                 int64_t stride, const int32_t *d_group, const double *d_w,
+                                 int nb, double *G, bool accumulate) {
+  if (np <= 0) return;
+  SGW_REQUIRE(d_group != nullptr || ngroup_ == 1, SGW_E_ARG,
+              "pw operator: a group index per vector is required with several groups");
+  const bool cubic = n1_ == n2_ && n2_ == n3_ && columns_contiguous_;
+  SGW_REQUIRE(cubic && (n1_ == 24 || n1_ == 32 || n1_ == 36 || n1_ == 48), SGW_E_ARG,
+              "pw operator: the V_loc gradient needs a cubic 24/32/36/48 box");
+  const size_t lds = sizeof(double2) * (nslot_ + (size_t)nc1_ * n1_ + (size_t)n1_ * n1_);
+  SGW_REQUIRE(lds <= 64 * 1024, SGW_E_ARG, "pw operator: gradient plane does not fit in LDS");
+  SGW_REQUIRE(nb >= 1 && nb < 65536, SGW_E_ARG, "pw operator: bad gradient partial count");
+  ensure_scratch(np);
+  PwDev d;
+  d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
+  d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
+  d.ld = ld; d.nproj = nproj_;
+  d.vloc = vloc_.p; d.npw = d_npw_.p;
+  d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
+  d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
+  d.col_lo = col_lo_.p;
+  d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
+  d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
+  d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
+  const dim3 gcol((unsigned)((nslot_ + 63) / 64), np);
+  auto run = [&](auto tag) {
+    constexpr int NB = decltype(tag)::value;
+    k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, x, stride, d_group, nullptr, T_.p);
+    k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, y, stride, d_group, nullptr, S_.p);
+    k_pw_plane_grad<NB><<<dim3(NB, nb), 64, lds, st>>>(d, T_.p, S_.p, np, d_w, G,
+                                                       accumulate ? 1 : 0);
+  };
+  switch (n1_) {
+    case 24: run(std::integral_constant<int, 24>{}); break;
+    case 32: run(std::integral_constant<int, 32>{}); break;
+    case 36: run(std::integral_constant<int, 36>{}); break;
+    default: run(std::integral_constant<int, 48>{}); break;
   }
   SGW_HIP(hipGetLastError());
 }

@@ -65,6 +65,7 @@ MODULE sgw_capi
     INTEGER(c_int32_t) :: lmax
     INTEGER(c_int32_t) :: max_iter
     REAL(c_double)     :: threshold
+    INTEGER(c_int32_t) :: shift_test   ! 0: seed residual only (reference); 1: every shift
   END TYPE sgw_batch_config
 
   INTERFACE

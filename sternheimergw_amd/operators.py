@@ -46,8 +46,11 @@ class Operator:
                                                  _lib.ptr(x), x.shape[1], _lib.ptr(ax)))
         return ax
 
-    def solve_batched(self, rhs, sigma, group=None, lmax=4, threshold=1e-4, max_iter=10000):
+    def solve_batched(self, rhs, sigma, group=None, lmax=4, threshold=1e-4, max_iter=10000,
+                      shift_test=False):
         """Batched multishift BiCGStab(l): rhs (nsys, n), sigma (nsys, nshift).
+        shift_test=False stops a system on the reference's seed-residual test;
+        True also requires every shifted residual estimate below threshold.
         Returns x (nsys, nshift, ld), ierr (nsys,), iters (nsys,)."""
         rhs = _lib.as_c128(np.atleast_2d(rhs))
         nsys = rhs.shape[0]
@@ -62,7 +65,7 @@ class Operator:
         x = np.zeros((nsys, nshift, ld), dtype=np.complex128)
         ierr = np.zeros(nsys, dtype=np.int32)
         iters = np.zeros(nsys, dtype=np.int32)
-        cfg = _lib.sgw_batch_config(nsys, nshift, ld, lmax, max_iter, threshold)
+        cfg = _lib.sgw_batch_config(nsys, nshift, ld, lmax, max_iter, threshold, int(shift_test))
         _lib.check(_lib.lib().sgw_sternheimer_solve_batched(
             self.handle, ctypes.byref(cfg), _lib.ptr(g), _lib.ptr(rhs), _lib.ptr(sigma),
             _lib.ptr(x), _lib.ptr(ierr), _lib.ptr(iters)))

@@ -81,6 +81,32 @@ def test_bicgstab_converged_parity(lin_prob, dense, ll):
     assert rel.max() <= TIGHT_PARITY
 
 
+@pytest.mark.parametrize("ll", [1, 2, 4])
+def test_batched_shift_test(lin_prob, dense, ll):
+    """The batched solver's shift test (no counterpart in the reference): with
+    the fixture's shifts reversed the seed is the easiest system, and the
+    solve must go on until every shifted residual passes.  Same iterations
+    and iterates as the oracle with the same test; two systems (b, i b).
+    The test reads the residuals off Frommer's collinearity factors; for
+    l >= 9 rounding in the recurrences breaks that relation (the oracle
+    itself ends at a true shifted residual of 1.6e-7 for l = 9 on this
+    problem), so it is exercised at the degrees where it holds."""
+    a, b = lin_prob["LINEAR_OPERATOR"], lin_prob["RIGHT_HAND_SIDE"]
+    sig = lin_prob["LIST_SHIFT"][::-1].copy()
+    thr = 1e-8
+    x, ierr, iters = dense.solve_batched(np.stack([b, 1j * b]), np.stack([sig, sig]), lmax=ll,
+                                         threshold=thr, shift_test=True)
+    assert np.all(ierr == 0) and iters[0] == iters[1]
+    for s, bs in enumerate((b, 1j * b)):
+        for k in range(sig.size):
+            assert _residual(a, sig[k], x[s, k], bs) < 1.01 * thr
+    xo, eo, io = orc.bicgstab(orc.DenseOp(a), b, sig, lmax=ll, threshold=thr, shift_test=True)
+    assert eo == 0 and abs(io - iters[0]) <= 1
+    if io == iters[0]:
+        rel = np.linalg.norm(x[0] - xo, axis=1) / np.linalg.norm(xo, axis=1)
+        assert rel.max() <= LOOSE_PARITY
+
+
 def test_bicgstab_callback_matches_device_operator(lin_prob, dense):
     """The host-callback entry (the reference's procedure argument) and the
     device operator give the same multishift solution."""

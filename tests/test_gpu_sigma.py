@@ -1,0 +1,105 @@
+"""GPU parity of the Sigma_c convolution GEMM (sgw_sigma_conv_device) and of
+its synthetic-input generator against the CPU oracle (oracle/sigma_oracle.py)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import sigma_oracle as so
+from sternheimergw_amd import sigma
+
+pytestmark = pytest.mark.gpu
+
+DEV = torch.device("cuda:0")
+# FP64 GEMM: the device sums K = nin * n products in a different order than
+# BLAS; the difference is a few ulps times sqrt(K).
+FP64_TOL = 1e-13
+
+
+def _to_dev_colmajor(m, ld):
+    """numpy (..., n, n) M[a, b] -> device (..., n, ld) with element (a, b) at [..., b, a]."""
+    n = m.shape[-1]
+    buf = np.zeros(m.shape[:-2] + (n, ld), dtype=np.complex128)
+    buf[..., :n] = np.swapaxes(m, -1, -2)
+    return torch.from_numpy(buf).to(DEV)
+
+
+def _from_dev_colmajor(t, n):
+    return np.swapaxes(t.cpu().numpy()[..., :n], -1, -2)
+
+
+def _rand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+@pytest.mark.parametrize("n,ld,nout,nin", [(200, 200, 3, 5), (130, 131, 2, 2), (17, 20, 1, 3),
+                                           (1, 1, 2, 1), (64, 64, 2, 4)])
+@pytest.mark.parametrize("accumulate", [False, True])
+def test_conv_matches_oracle(n, ld, nout, nin, accumulate):
+    print(f"sigma_conv n={n} ld={ld} nout={nout} nin={nin} acc={accumulate}", flush=True)
+    rng = np.random.Generator(np.random.PCG64(n + 7 * nin))
+    G, W = _rand(rng, nout, nin, n, n), _rand(rng, nin, n, n)
+    wt = rng.uniform(0.1, 1.0, nin)
+    alpha = complex(-0.4, 0.9)
+    S0 = _rand(rng, nout, n, n)
+    ref = so.sigma_conv(G, W, wt, alpha) + (S0 if accumulate else 0)
+    d_g, d_w = _to_dev_colmajor(G, ld), _to_dev_colmajor(W, ld)
+    d_s = _to_dev_colmajor(S0 if accumulate else np.zeros_like(S0), ld)
+    d_wt = torch.from_numpy(wt).to(DEV)
+    sigma.conv_device(d_g, d_w, d_wt, d_s, n, ld, nout, nin, alpha, accumulate=accumulate)
+    torch.cuda.synchronize()
+    got = _from_dev_colmajor(d_s, n)
+    for i in range(nout):
+        err = np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i])
+        assert err <= FP64_TOL, (i, err)
+    # padding rows between n and ld stay untouched
+    if ld > n:
+        pad = d_s.cpu().numpy()[..., n:]
+        assert np.all(pad == 0)
+
+
+def test_synthetic_inputs_match_oracle():
+    n, ld = 77, 80
+    w = sigma.sigma_workload(n=n, nfreq=3)
+    d_g = torch.zeros((2, w.nin, n, ld), dtype=torch.complex128, device=DEV)
+    d_w = torch.zeros((w.nin, n, ld), dtype=torch.complex128, device=DEV)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
+    sigma.synthetic_device(w, 1, 2, d_g, d_w, t(w.nu), t(w.gvec), t(w.theta), t(w.cw), ld)
+    torch.cuda.synchronize()
+    G = _from_dev_colmajor(d_g, n)
+    W = _from_dev_colmajor(d_w, n)
+    for il in range(2):
+        for j in range(w.nin):
+            ref = so.gen_g(n, 1 + il, j, w.nout, w.nin, w.seed, w.nu[1 + il, j])
+            assert np.allclose(G[il, j], ref, rtol=1e-15, atol=0)
+    for j in range(w.nin):
+        ref = so.gen_w(w.gvec, w.theta, w.cw[j], w.length)
+        assert np.abs(W[j] - ref).max() <= 1e-15 * np.abs(ref).max()
+
+
+def test_full_size_sampled_entries():
+    """N_G = 1500, 64 convolution frequencies, one output frequency: every
+    sampled entry of Sigma equals its FP64 CPU recomputation from the very
+    G and W the device generated."""
+    w = sigma.sigma_workload()
+    n, ld, i = w.n, w.n, 5
+    print("sigma full size: generating", flush=True)
+    d_g = torch.empty((1, w.nin, n, ld), dtype=torch.complex128, device=DEV)
+    d_w = torch.empty((w.nin, n, ld), dtype=torch.complex128, device=DEV)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(DEV)
+    sigma.synthetic_device(w, i, 1, d_g, d_w, t(w.nu), t(w.gvec), t(w.theta), t(w.cw), ld)
+    d_s = torch.zeros((1, n, ld), dtype=torch.complex128, device=DEV)
+    print("sigma full size: convolution", flush=True)
+    sigma.conv_device(d_g, d_w, t(w.weight), d_s, n, ld, 1, w.nin, w.alpha)
+    torch.cuda.synchronize()
+    rng = np.random.Generator(np.random.PCG64(11))
+    rows = rng.integers(0, n, 24)
+    cols = rng.integers(0, n, 24)
+    S = d_s[0].cpu().numpy()  # [b, a]
+    for a, b in zip(rows, cols):
+        grow = d_g[0, :, :, a].cpu().numpy()      # (nin, n): G_j(a, c) at [j, c]
+        wcol = d_w[:, b, :n].cpu().numpy()        # (nin, n): W_j(c, b) at [j, c]
+        ref = w.alpha * np.sum(w.weight * np.sum(grow * wcol, axis=1))
+        assert abs(S[b, a] - ref) <= 1e-13 * max(abs(ref), 1e-300), (a, b)
+    # the generated inputs themselves are the oracle's
+    g0 = so.gen_g(n, i, 3, w.nout, w.nin, w.seed, w.nu[i, 3])[:, :8]
+    assert np.allclose(d_g[0, 3, :8, :n].cpu().numpy().T, g0, rtol=1e-15, atol=0)

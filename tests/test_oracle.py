@@ -65,6 +65,25 @@ def test_bicgstab_direct_solve(lin_prob):
         assert np.linalg.norm(x[k] - ex) <= 1e-10 * np.linalg.norm(ex)
 
 
+def test_shift_test_covers_every_shift(lin_prob):
+    """With the easiest shift as seed (the fixture's shifts reversed), the
+    reference's seed-only test stops while most shifted residuals are still
+    large; the shift test (seed residual times |1/(theta phi)| for every
+    shift) keeps iterating until each true shifted residual is below the
+    threshold."""
+    a, b = lin_prob["LINEAR_OPERATOR"], lin_prob["RIGHT_HAND_SIDE"]
+    sig = lin_prob["LIST_SHIFT"][::-1].copy()
+    thr = 1e-8
+    op = orc.DenseOp(a)
+    res = {}
+    for st in (False, True):
+        x, ierr, it = orc.bicgstab(op, b, sig, lmax=4, threshold=thr, shift_test=st)
+        assert ierr == 0
+        res[st] = np.array([np.linalg.norm(b - (a @ x[k] + sig[k] * x[k])) for k in range(sig.size)])
+    assert res[False][0] < thr and (res[False] > thr).sum() > 10
+    assert res[True].max() < thr
+
+
 @pytest.mark.parametrize("ll", [1, 2, 4, 7])
 def test_bicgstab_collinear_residuals(ll):
     """Frommer's invariant: every shifted residual b - (A + sigma_k) x_k stays
