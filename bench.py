@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--shift-test", type=int, default=1,
                     help="1: a system stops when every shift's residual estimate passes the "
                          "threshold; 0: the reference's test on the seed residual alone")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="mixed: FP32 local-potential FFTs once a system's residual estimate "
+                         "is below --relax-threshold (vectors, recurrences and test in FP64)")
+    ap.add_argument("--relax-threshold", type=float, default=1e-4)
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample of the batch (0 to skip)")
     ap.add_argument("--kernel-timing", type=int, default=1,
@@ -58,6 +62,8 @@ def parse():
     ap.add_argument("--workload", choices=("solve", "sigma"), default="solve",
                     help="solve: the batched Sternheimer solve (headline); sigma: the Sigma_c "
                          "frequency convolution GEMM (N_G = 1500, 64 x 64 frequencies)")
+    ap.add_argument("--sigma-precision", choices=("fp64", "fp32s"), default="fp64",
+                    help="fp32s: FP32 MFMA products, per-frequency FP32 sums, FP64 frequency sum")
     ap.add_argument("--sigma-batch", type=int, default=8,
                     help="output frequencies whose Green's-function stack is resident at once")
     return ap.parse_args()
@@ -78,6 +84,10 @@ def run_sigma(args):
     dev = torch.device("cuda:0")
     w = sigma.sigma_workload()
     n, ld, nb = w.n, w.n, max(1, min(args.sigma_batch, w.nout))
+    fp32s = args.sigma_precision == "fp32s"
+    prec = sigma.SIGMA_FP32_SPLIT if fp32s else sigma.SIGMA_FP64
+    kname = "sigma_f32s" if fp32s else "sigma_f64"
+    peak = 157.3 if fp32s else 78.6  # dense FP32 / FP64 MFMA, TFLOP/s
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
     d_nu, d_gvec, d_theta, d_cw, d_wt = t(w.nu), t(w.gvec), t(w.theta), t(w.cw), t(w.weight)
     d_w = torch.empty((w.nin, n, ld), dtype=torch.complex128, device=dev)
@@ -88,7 +98,8 @@ def run_sigma(args):
     def batch(i0):
         ni = min(nb, w.nout - i0)
         sigma.synthetic_device(w, i0, ni, d_g, None, d_nu, d_gvec, d_theta, d_cw, ld)
-        sigma.conv_device(d_g, d_w, d_wt, d_s[i0:i0 + ni], n, ld, ni, w.nin, w.alpha)
+        sigma.conv_device(d_g, d_w, d_wt, d_s[i0:i0 + ni], n, ld, ni, w.nin, w.alpha,
+                          precision=prec)
         return ni
 
     for _ in range(args.warmup):
@@ -105,7 +116,7 @@ def run_sigma(args):
         print(f"step {s}: {time.perf_counter() - t0:.2f} s", flush=True)
     elapsed = time.perf_counter() - t0
     ms, cnt = ctypes.c_double(), ctypes.c_int64()
-    _lib.check(L.sgw_kernel_timing_query(b"sigma_f64", ctypes.byref(ms), ctypes.byref(cnt)))
+    _lib.check(L.sgw_kernel_timing_query(kname.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
     _lib.check(L.sgw_kernel_timing(0))
     flops = 8.0 * n ** 3 * w.nout * w.nin * args.steps
     conv_s = ms.value / 1e3
@@ -133,15 +144,16 @@ def run_sigma(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c64 products, c128 frequency sum" if fp32s else "c128",
         "data": "synthetic (seeded hash G stack generated on the device; Gaussian-kernel W)",
-        "config": {"workload": "sigma-conv-fp64: N_G 1500, 64 output x 64 convolution "
-                               "frequencies, FP64 MFMA",
+        "config": {"workload": ("sigma-conv-fp32split" if fp32s else "sigma-conv-fp64")
+                               + ": N_G 1500, 64 output x 64 convolution frequencies, "
+                               + ("FP32 MFMA" if fp32s else "FP64 MFMA"),
                    "n": n, "nout": w.nout, "nin": w.nin, "batch": nb,
                    "max_rel_err_sampled_vs_fp64_cpu": worst,
                    "wall_tflops_incl_generation": round(flops / elapsed / 1e12, 3)},
-        "roofline": {"kernel": "k_sigma_f64", "bound": "mfma", "achieved": round(achieved, 3),
-                     "peak": 78.6, "unit": "TFLOP/s", "frac": round(achieved / 78.6, 4),
+        "roofline": {"kernel": "k_" + kname, "bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
                      "traffic": None, "launches": cnt.value,
                      "avg_launch_ms": round(ms.value / max(cnt.value, 1), 3),
                      "flops_per_launch": 8.0 * n ** 3 * nb * w.nin},
@@ -226,8 +238,9 @@ def main():
     h_sigma = np.ascontiguousarray(w.sigma)
     d_group = torch.from_numpy(w.group).to(dev)
     d_x = torch.zeros((nsys, nshift, ld), dtype=torch.complex128, device=dev)
+    relax = args.relax_threshold if args.precision == "mixed" else 0.0
     cfg = _lib.sgw_batch_config(nsys, nshift, ld, args.lmax, 10000, args.threshold,
-                                args.shift_test)
+                                args.shift_test, relax)
     solver = L.sgw_batch_solver_create(op.handle, ctypes.byref(cfg))
     if not solver:
         raise RuntimeError(L.sgw_last_error().decode())
@@ -319,7 +332,7 @@ def main():
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
         if prof.get("workload") == [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold,
-                                    args.shift_test]:
+                                    args.shift_test, relax]:
             traffic = prof["traffic_bytes_per_happly"]
             traffic_note = {"source": "profiles/happly_traffic.json",
                             "algorithmic_bytes_per_launch": round(bh * total_vec_happly
@@ -338,13 +351,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128",
+        "dtype": "c128" if relax == 0 else "c128 (c64 local-potential FFTs once relaxed)",
         "data": "synthetic (seeded fcc cell, V_loc, beta projectors; FP64 diagonalisation)",
         "config": {
-            "workload": "batched-fp64-48^3: 16 bands x 16 q x 32 shifts, 25 Ha, BiCGStab(4)",
+            "workload": ("batched-fp64-48^3" if relax == 0 else "batched-mixed-48^3")
+                        + ": 16 bands x 16 q x 32 shifts, 25 Ha, BiCGStab(4)",
             "box": list(w.box), "nsys": nsys, "nshift": nshift,
             "npw_min_max": [int(w.npw.min()), int(w.npw.max())], "nproj": int(w.nproj),
             "threshold": args.threshold, "lmax": args.lmax, "shift_test": args.shift_test,
+            "relax_threshold": relax,
             "solves_per_step": nsys * nshift, "solves_meeting_1e-8": good,
             "max_true_residual": float(res.max()),
             "iterations_max": int(iters.max()), "iterations_median": float(np.median(iters)),

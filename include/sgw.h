@@ -133,6 +133,12 @@ sgw_operator *sgw_operator_pw(const sgw_pw_desc *desc);
 
 void sgw_operator_destroy(sgw_operator *op);
 
+/* Precision of a plane-wave operator's local-potential FFTs in every later
+ * application: 0 FP64 (default), 1 FP32 (input, output and the other terms
+ * stay FP64).  No counterpart in the reference (FP64 only); used to measure
+ * and test the FP32 pipeline that the mixed-precision solve relaxes to. */
+int sgw_operator_set_fft_precision(sgw_operator *op, int32_t mode);
+
 /* ax[v] = (A_{group[v]} + sigma[v]) x[v], v < nvec; host arrays with leading
  * dimension ldx (>= operator ld).  group may be NULL for one-group operators. */
 int sgw_operator_apply(sgw_operator *op, int32_t nvec, const int32_t *group,
@@ -172,6 +178,12 @@ typedef struct {
                           |1/(theta phi)| ||r_seed|| (no counterpart in
                           the reference, which can leave a shift above
                           the threshold)                               */
+  double relax_threshold; /* 0: FP64 throughout.  > 0: mixed precision
+                          (plane-wave operators): once a system's
+                          residual estimate is below relax_threshold its
+                          local-potential FFTs run in FP32 (relaxed
+                          matrix-vector products); vectors, recurrences
+                          and the convergence test stay FP64             */
 } sgw_batch_config;
 
 typedef struct sgw_batch_solver sgw_batch_solver;
@@ -234,6 +246,10 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
  * sigma.f90:417-523); here it is the G'' contraction in reciprocal space.
  * ------------------------------------------------------------------------- */
 #define SGW_SIGMA_FP64 0   /* FP64 operands, FP64 MFMA, FP64 accumulation */
+#define SGW_SIGMA_FP32_SPLIT 1  /* operands rounded to FP32, FP32 MFMA; each
+                                   frequency's sum FP32, the sum over
+                                   frequencies FP64 (no counterpart in the
+                                   reference, which is FP64) */
 
 typedef struct {
   int32_t n;

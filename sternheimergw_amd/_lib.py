@@ -59,11 +59,8 @@ class sgw_pw_desc(ctypes.Structure):
 
 class sgw_batch_config(ctypes.Structure):
     _fields_ = [("nsys", c_int32), ("nshift", c_int32), ("ld", c_int64), ("lmax", c_int32),
-                ("max_iter", c_int32), ("threshold", c_double), ("shift_test", c_int32)]
-
-
-class sgw_complex(ctypes.Structure):
-    _fields_ = [("re", c_double), ("im", c_double)]
+                ("max_iter", c_int32), ("threshold", c_double), ("shift_test", c_int32),
+                ("relax_threshold", c_double)]
 
 
 class sgw_sigma_conv_desc(ctypes.Structure):
@@ -73,6 +70,7 @@ class sgw_sigma_conv_desc(ctypes.Structure):
 
 
 SIGMA_FP64 = 0
+SIGMA_FP32_SPLIT = 1
 
 # (name, restype, argtypes) of every exported entry point of include/sgw.h
 SIGNATURES = [
@@ -86,6 +84,7 @@ SIGNATURES = [
     ("sgw_operator_callback", c_void_p, [c_void_p, c_void_p, c_int64]),
     ("sgw_operator_pw", c_void_p, [ctypes.POINTER(sgw_pw_desc)]),
     ("sgw_operator_destroy", None, [c_void_p]),
+    ("sgw_operator_set_fft_precision", c_int32, [c_void_p, c_int32]),
     ("sgw_operator_apply", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p, c_int64,
                                      c_void_p]),
     ("sgw_operator_apply_device", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p,

@@ -22,10 +22,10 @@ class BatchSolver:
     """A batched multishift BiCGStab(l) solver bound to one operator."""
 
     def __init__(self, op, nsys, nshift, ld, lmax=4, max_iter=10000, threshold=1e-8,
-                 shift_test=False):
+                 shift_test=False, relax_threshold=0.0):
         self.op = op
         self.cfg = _lib.sgw_batch_config(nsys, nshift, ld, lmax, max_iter, threshold,
-                                         int(shift_test))
+                                         int(shift_test), relax_threshold)
         self.handle = _lib.lib().sgw_batch_solver_create(op.handle, ctypes.byref(self.cfg))
         if not self.handle:
             raise _lib.SgwError(-1, _lib.lib().sgw_last_error().decode())

@@ -111,7 +111,7 @@ extern "C" int sgw_sternheimer_adjoint(sgw_batch_solver *solver, const int32_t *
       KScope ks("adjoint_solve", st);
       solver->engine->solve(*solver->op->op, st, cv, ld, sig_c.p, h_conj.data(), d_group,
                             c.threshold, c.max_iter, y, ld, ierr.data(), it.data(), 1,
-                            c.shift_test);
+                            c.shift_test, c.relax_threshold);
     }
     int worst = 0;
     for (int s = 0; s < S; ++s) worst = std::max(worst, ierr[s]);

@@ -22,7 +22,8 @@ class BatchBicgstab {
   void solve(Operator &op, hipStream_t st, const double2 *d_rhs, int64_t rhs_ld,
              const double2 *d_sigma, const double2 *h_sigma, const int32_t *d_group,
              double threshold, int max_iter, double2 *d_x, int64_t x_ld, int32_t *h_ierr,
-             int32_t *h_iters, int check_every = 1, int shift_test = 0);
+             int32_t *h_iters, int check_every = 1, int shift_test = 0,
+             double relax_thr = 0.0);
 
  private:
   struct Impl;

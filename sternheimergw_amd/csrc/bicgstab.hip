@@ -18,6 +18,7 @@
 #include "sgw_internal.h"
 #include "bicgstab.h"
 #include "ktimer.h"
+#include "operators.h"
 
 #include <algorithm>
 #include <cmath>
@@ -449,7 +450,7 @@ __global__ __launch_bounds__(NT) void k_mr_mgs(const int32_t *__restrict__ activ
 __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restrict__ in,
                         int32_t *__restrict__ out, const double *__restrict__ norm,
                         int32_t *__restrict__ iters, int32_t *__restrict__ state, Scal sc,
-                        int nsh, int shift_test) {
+                        int nsh, int shift_test, double relax_thr, int32_t *__restrict__ relax) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
   if (!in[s]) {
@@ -465,6 +466,10 @@ __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restric
     }
     nr *= zmax;
   }
+  // relaxed matrix-vector products (inexact Krylov): once the residual is
+  // small the error of an FP32 local-potential FFT, proportional to the
+  // vector it multiplies, stays below the final tolerance
+  if (relax && nr < relax_thr) relax[s] = 1;
   int keep = 1;
   if (nr < thr) {
     keep = 0;
@@ -657,12 +662,15 @@ struct BatchBicgstab::Impl {
   const int32_t *group_dev = nullptr;
 
   int shift_test = 0;
+  double relax_thr = 0.0;
+  DevBuf<int32_t> relax;
 
   // ||r_0|| test on the systems of `in`; survivors go to `out`
   void check(hipStream_t st, int iter, double thr, int32_t *in, int32_t *out) {
     k_norm<<<nsys, kBlock, 0, st>>>(nsys, in, R.p, (l + 1) * ld, ld, norms.p);
     k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, in, out, norms.p, iters.p,
-                                               state.p, sc, nsh, shift_test);
+                                               state.p, sc, nsh, shift_test, relax_thr,
+                                               relax_thr > 0 ? relax.p : nullptr);
   }
 };
 
@@ -674,10 +682,27 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
                           const double2 *d_sigma, const double2 *h_sigma,
                           const int32_t *d_group, double threshold, int max_iter,
                           double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
-                          int check_every, int shift_test) {
+                          int check_every, int shift_test, double relax_thr) {
   Impl &m = *impl_;
   m.shift_test = shift_test;
   const int S = m.nsys, l = m.l;
+  // relaxed FP32 local-potential FFTs per system (plane-wave operators only)
+  auto *pw = dynamic_cast<PwOperator *>(&op);
+  m.relax_thr = (pw != nullptr && relax_thr > 0) ? relax_thr : 0.0;
+  struct RelaxGuard {
+    PwOperator *pw;
+    int mode;
+    const int32_t *flags;
+    ~RelaxGuard() {
+      if (pw) pw->set_fft_precision(mode, flags);
+    }
+  } guard{nullptr, 0, nullptr};
+  if (m.relax_thr > 0) {
+    if (m.relax.n < (size_t)S) m.relax.alloc(S);
+    m.relax.zero(st);
+    guard = RelaxGuard{pw, pw->fft_precision(), nullptr};
+    pw->set_fft_precision(2, m.relax.p);
+  }
   SGW_REQUIRE(d_group != nullptr || op.ngroups() == 1, SGW_E_ARG,
               "bicgstab: a group index per system is required with several groups");
   {  // every group index must address an operator group

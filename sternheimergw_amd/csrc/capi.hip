@@ -247,6 +247,17 @@ sgw_operator *sgw_operator_pw(const sgw_pw_desc *desc) {
 
 void sgw_operator_destroy(sgw_operator *op) { delete op; }
 
+int sgw_operator_set_fft_precision(sgw_operator *op, int32_t mode) {
+  return guarded([&] {
+    SGW_REQUIRE(op && op->op, SGW_E_ARG, "operator_set_fft_precision: null operator");
+    SGW_REQUIRE(mode == 0 || mode == 1, SGW_E_ARG, "operator_set_fft_precision: mode 0 or 1");
+    auto *pw = dynamic_cast<PwOperator *>(op->op.get());
+    SGW_REQUIRE(pw != nullptr, SGW_E_ARG, "operator_set_fft_precision: plane-wave operators only");
+    pw->set_fft_precision(mode, nullptr);
+    return 0;
+  });
+}
+
 int sgw_operator_apply_device(sgw_operator *op, int32_t nvec, const int32_t *d_group,
                               const sgw_complex *d_sigma, const sgw_complex *d_x, int64_t ldx,
                               sgw_complex *d_ax, void *stream) {
@@ -334,7 +345,7 @@ int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
     std::vector<int32_t> ierr(c.nsys), it(c.nsys);
     solver->engine->solve(*solver->op->op, (hipStream_t)stream, D2(d_rhs), c.ld, D2(d_sigma),
                           D2(h_sigma), d_group, c.threshold, c.max_iter, D2(d_x), c.ld,
-                          ierr.data(), it.data(), 1, c.shift_test);
+                          ierr.data(), it.data(), 1, c.shift_test, c.relax_threshold);
     int worst = 0;
     for (int s = 0; s < c.nsys; ++s) worst = std::max(worst, ierr[s]);
     if (h_ierr) std::memcpy(h_ierr, ierr.data(), sizeof(int32_t) * c.nsys);
@@ -368,7 +379,7 @@ int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
     BatchBicgstab eng(S, K, cfg->lmax, ld);
     std::vector<int32_t> e(S), it(S);
     eng.solve(*op->op, st, d_rhs.p, ld, d_sig.p, D2(sigma), d_g.p, cfg->threshold, cfg->max_iter,
-              d_x.p, ld, e.data(), it.data(), 1, cfg->shift_test);
+              d_x.p, ld, e.data(), it.data(), 1, cfg->shift_test, cfg->relax_threshold);
     SGW_HIP(hipMemcpy(x, d_x.p, sizeof(double2) * S * K * ld, hipMemcpyDeviceToHost));
     int worst = 0;
     for (int s = 0; s < S; ++s) worst = std::max(worst, e[s]);

@@ -63,11 +63,21 @@ class PwOperator : public Operator {
   void grad_accumulate(hipStream_t st, int np, const double2 *x, const double2 *y,
                        int64_t stride, const int32_t *d_group, const double *d_w, int nb,
                        double *G, bool accumulate);
+  // Precision of the local-potential FFT pipeline: 0 FP64, 1 FP32, 2 per
+  // vector, FP32 where d_relax[v] != 0 (a device array of the caller, read at
+  // every apply; the cubic FFT path only, else FP64).
+  void set_fft_precision(int mode, const int32_t *d_relax) {
+    fft_mode_ = mode;
+    relax_ = d_relax;
+  }
+  int fft_precision() const { return fft_mode_; }
   int box_n() const { return n1_; }
   int64_t column_slots() const { return nslot_; }
   int64_t box_size() const { return (int64_t)n1_ * n2_ * n3_; }
 
  private:
+  int fft_mode_ = 0;
+  const int32_t *relax_ = nullptr;
   void ensure_scratch(int nvec);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;
@@ -77,7 +87,7 @@ class PwOperator : public Operator {
   int n1_, n2_, n3_, ngroup_, nproj_;
   int M1_, M2_, nc1_, nc2_, nslot_;
   std::vector<int32_t> npw_;
-  DevBuf<double> vloc_, kin_, dcoef_;
+  DevBuf<double> vloc_, vloc_zxy_, kin_, dcoef_;
   DevBuf<int32_t> d_npw_, col_start_, col_cnt_, col_lo_, sidx_, sm3_, slot_of_, m3_of_;
   DevBuf<double2> proj_, tw_, S_, T_, loc_;
   bool columns_contiguous_ = true;

@@ -56,6 +56,7 @@ struct PwDev {
   int64_t ld;
   int nproj;
   const double *vloc;          // [n3][n2][n1]
+  const double *vloc_zxy;      // [n3][n1][n2]: lanes of the x stage (one per y) read rows
   const int32_t *npw;          // [ngroup]
   const int32_t *col_start;    // [ngroup][nslot]
   const int32_t *col_cnt;      // [ngroup][nslot]
@@ -157,35 +158,55 @@ __global__ __launch_bounds__(kBlock) void k_pw_plane(PwDev d, const double2 *__r
   }
 }
 
+// The local-potential FFT pipeline runs in the element type C: double2, or
+// float2 for vectors whose solve has been relaxed to FP32 (input and output
+// stay FP64).  With a flag array, a kernel instance handles the vectors whose
+// flag relax[v] equals `want`; without one, every vector.  The FP32 scratch
+// of vector v sits in the first half of its FP64 slot.
+template <typename C> __device__ inline C to_c(double2 v);
+template <> __device__ inline double2 to_c<double2>(double2 v) { return v; }
+template <> __device__ inline float2 to_c<float2>(double2 v) {
+  return make_float2((float)v.x, (float)v.y);
+}
+template <typename C> __device__ inline C czero() { return CplxT<C>::make(0, 0); }
+template <typename C> constexpr int kSlotScale = (int)(sizeof(double2) / sizeof(C));
+
+__device__ inline bool precision_skip(const int32_t *__restrict__ relax, int want, int v) {
+  return relax != nullptr && relax[v] != want;
+}
+
 // z inverse transform of the sphere columns: T[v][z][c] = sum_m3 x(c, m3)
 // e^{+2 pi i m3 z / N}.  One lane per column (m1, m2); the column's plane
 // waves are consecutive in sorted order with m3 = lo .. lo + cnt - 1 (the
 // sphere is convex).
-template <int N>
+template <int N, typename C = double2>
 __global__ __launch_bounds__(64) void k_pw_zinv(PwDev d, const double2 *__restrict__ in,
                                                 int64_t sin, const int32_t *__restrict__ group,
                                                 const int32_t *__restrict__ active,
-                                                double2 *__restrict__ T) {
+                                                C *__restrict__ T,
+                                                const int32_t *__restrict__ relax = nullptr,
+                                                int want = 0) {
   const int v = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
   if (active && !active[v]) return;
+  if (precision_skip(relax, want, v)) return;
   if (c >= d.nslot) return;
   const int g = group ? group[v] : 0;
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
-  double2 *Tv = T + (int64_t)v * N * d.nslot + c;
+  C *Tv = T + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
   if (cnt == 0) {
 #pragma unroll
-    for (int z = 0; z < N; ++z) Tv[(int64_t)z * d.nslot] = mk(0, 0);
+    for (int z = 0; z < N; ++z) Tv[(int64_t)z * d.nslot] = czero<C>();
     return;
   }
   const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
   const double2 *x = in + v * sin;
-  double2 buf[N];
+  C buf[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const int m = i < N / 2 ? i : i - N;
     const int k = m - lo;
-    buf[i] = (k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
+    buf[i] = (k >= 0 && k < cnt) ? to_c<C>(x[sidx[k]]) : czero<C>();
   }
   reg_fft<N, +1>(buf);
 #pragma unroll
@@ -195,21 +216,24 @@ __global__ __launch_bounds__(64) void k_pw_zinv(PwDev d, const double2 *__restri
 // z forward transform of the sphere columns back to the plane waves:
 // loc[v][i] = (1/Nbox) sum_z S[v][z][c] e^{-2 pi i m3 z / N} for the plane
 // waves i of column c.
-template <int N>
-__global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const double2 *__restrict__ S,
+template <int N, typename C = double2>
+__global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S,
                                                 const int32_t *__restrict__ group,
                                                 const int32_t *__restrict__ active,
-                                                double2 *__restrict__ loc) {
+                                                double2 *__restrict__ loc,
+                                                const int32_t *__restrict__ relax = nullptr,
+                                                int want = 0) {
   const int v = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
   if (active && !active[v]) return;
+  if (precision_skip(relax, want, v)) return;
   if (c >= d.nslot) return;
   const int g = group ? group[v] : 0;
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
   if (cnt == 0) return;
   const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
-  const double2 *Sv = S + (int64_t)v * N * d.nslot + c;
-  double2 buf[N];
+  const C *Sv = S + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
+  C buf[N];
 #pragma unroll
   for (int z = 0; z < N; ++z) buf[z] = Sv[(int64_t)z * d.nslot];
   reg_fft<N, -1>(buf);
@@ -219,7 +243,7 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const double2 *__restri
   for (int i = 0; i < N; ++i) {
     const int m = i < N / 2 ? i : i - N;
     const int k = m - lo;
-    if (k >= 0 && k < cnt) lv[sidx[k]] = mk(buf[i].x * sc, buf[i].y * sc);
+    if (k >= 0 && k < cnt) lv[sidx[k]] = mk((double)buf[i].x * sc, (double)buf[i].y * sc);
   }
 }
 
@@ -233,29 +257,34 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const double2 *__restri
 //      columns' values S[v][c][z].
 // LDS holds the column values T[nc1][nc2] and one row buffer P[nc1][N] that
 // the X stage overwrites column by column (each lane owns its column y).
-template <int N>
-__global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const double2 *__restrict__ Tin,
+template <int N, typename C = double2>
+__global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const C *__restrict__ Tin,
                                                      const int32_t *__restrict__ group,
                                                      const int32_t *__restrict__ active,
-                                                     double2 *__restrict__ S) {
-  extern __shared__ double2 smem[];
+                                                     C *__restrict__ S,
+                                                     const int32_t *__restrict__ relax = nullptr,
+                                                     int want = 0) {
+  using R = typename CplxT<C>::S;
+  extern __shared__ double2 smem_d[];
+  C *smem = reinterpret_cast<C *>(smem_d);
   const int z = blockIdx.x, v = blockIdx.y;
   if (active && !active[v]) return;
+  if (precision_skip(relax, want, v)) return;
   const int g = group ? group[v] : 0;
   const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M1 = d.M1, M2 = d.M2;
   const int lane = threadIdx.x;
-  double2 *T = smem;          // nslot
-  double2 *P = T + nslot;     // nc1 * N
+  C *T = smem;          // nslot
+  C *P = T + nslot;     // nc1 * N
   const int32_t *cc = d.col_cnt + (int64_t)g * nslot;
-  const double2 *Tz = Tin + ((int64_t)v * N + z) * nslot;
+  const C *Tz = Tin + ((int64_t)v * N * kSlotScale<C> + z) * nslot;
   for (int c = lane; c < nslot; c += 64) T[c] = Tz[c];
   __syncthreads();
-  double2 buf[N];
+  C buf[N];
   if (lane < nc1) {  // Y inverse of row a
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int m = i < N / 2 ? i : i - N;
-      buf[i] = (m >= -M2 && m <= M2) ? T[lane * nc2 + m + M2] : mk(0, 0);
+      buf[i] = (m >= -M2 && m <= M2) ? T[lane * nc2 + m + M2] : czero<C>();
     }
     reg_fft<N, +1>(buf);
 #pragma unroll
@@ -267,14 +296,14 @@ __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const double2 *__r
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int m = i < N / 2 ? i : i - N;
-      buf[i] = (m >= -M1 && m <= M1) ? P[(m + M1) * N + y] : mk(0, 0);
+      buf[i] = (m >= -M1 && m <= M1) ? P[(m + M1) * N + y] : czero<C>();
     }
     reg_fft<N, +1>(buf);
-    const double *vrow = d.vloc + ((int64_t)z * N + y) * N;
+    const double *vcol = d.vloc_zxy + (int64_t)z * N * N + y;  // V(x, y, z) at vcol[x * N]
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-      const double vv = vrow[i];
-      buf[i] = mk(buf[i].x * vv, buf[i].y * vv);
+      const R vv = (R)vcol[i * N];
+      buf[i] = CplxT<C>::make(buf[i].x * vv, buf[i].y * vv);
     }
     reg_fft<N, -1>(buf);
 #pragma unroll
@@ -288,7 +317,7 @@ __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const double2 *__r
 #pragma unroll
     for (int i = 0; i < N; ++i) buf[i] = P[lane * N + i];
     reg_fft<N, -1>(buf);
-    double2 *Sz = S + ((int64_t)v * N + z) * nslot;
+    C *Sz = S + ((int64_t)v * N * kSlotScale<C> + z) * nslot;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       const int m = i < N / 2 ? i : i - N;
@@ -666,6 +695,15 @@ PwOperator::PwOperator(const PwDesc &desc) {
   }
   const int64_t nbox = (int64_t)n1_ * n2_ * n3_;
   vloc_.alloc(nbox); vloc_.upload(desc.vloc, nbox);
+  {  // [z][x][y] copy for the x stage of the plane kernels
+    std::vector<double> t((size_t)nbox);
+    for (int z = 0; z < n3_; ++z)
+      for (int y = 0; y < n2_; ++y)
+        for (int x = 0; x < n1_; ++x)
+          t[((size_t)z * n1_ + x) * n2_ + y] = desc.vloc[((size_t)z * n2_ + y) * n1_ + x];
+    vloc_zxy_.alloc(nbox);
+    vloc_zxy_.upload(t.data(), nbox);
+  }
   d_npw_.alloc(ngroup_); d_npw_.upload(npw_.data(), ngroup_);
   col_start_.alloc(col_start.size()); col_start_.upload(col_start.data(), col_start.size());
   col_cnt_.alloc(col_cnt.size()); col_cnt_.upload(col_cnt.data(), col_cnt.size());
@@ -691,6 +729,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
 
 int64_t PwOperator::group_n(int g) const { return npw_.at(g); }
 
+
 void PwOperator::ensure_scratch(int nvec) {
   size_t need = (size_t)nvec * nslot_ * n3_;
   if (S_.n < need) S_.alloc(need);
@@ -710,7 +749,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
   d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
   d.ld = ld; d.nproj = nproj_;
-  d.vloc = vloc_.p; d.npw = d_npw_.p;
+  d.vloc = vloc_.p; d.vloc_zxy = vloc_zxy_.p; d.npw = d_npw_.p;
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
@@ -721,19 +760,47 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   const size_t lds_fft = sizeof(double2) * (nslot_ + (size_t)nc1_ * n1_);
   const dim3 gcol((unsigned)((nslot_ + 63) / 64), nvec);
   const double2 *locv = nullptr;
+  SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
+              "pw operator: prepare() was not called for this batch of vectors");
+  const int nchunk = (int)((ld + kChunk - 1) / kChunk);
+  if (nproj_ > 0) {
+    // projector contraction becp = D proj^H x (a concurrent stream for it was
+    // measured: no gain, the plane kernel already fills the device)
+    KScope ks("pw_becp", st, 2);
+    k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
+        d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
+        part_.p);
+    k_pw_becp_reduce<<<dim3((unsigned)((nproj_ * kVT + 63) / 64), ngroup_, plan_tiles_), 64, 0,
+                       st>>>(d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
+                             part_.p, becp_.p);
+    SGW_HIP(hipGetLastError());
+  }
   auto launch_fft = [&](auto tag) {
     constexpr int NB = decltype(tag)::value;
+    // fft_mode_: 0 FP64, 1 FP32, 2 per vector from the relax flags
+    const int32_t *rl = fft_mode_ == 2 ? relax_ : nullptr;
+    float2 *T32 = reinterpret_cast<float2 *>(T_.p), *S32 = reinterpret_cast<float2 *>(S_.p);
+    const bool run64 = fft_mode_ != 1, run32 = fft_mode_ != 0;
     {
       KScope ks("pw_zinv", st, 2);
-      k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T_.p);
+      if (run64) k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T_.p, rl, 0);
+      if (run32)
+        k_pw_zinv<NB, float2><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T32, rl, 1);
     }
     {
       KScope ks("pw_plane", st, 2);
-      k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, T_.p, d_group, d_active, S_.p);
+      if (run64)
+        k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, T_.p, d_group, d_active,
+                                                                 S_.p, rl, 0);
+      if (run32)
+        k_pw_plane_fft<NB, float2><<<dim3(NB, nvec), 64, lds_fft / 2, st>>>(
+            d, T32, d_group, d_active, S32, rl, 1);
     }
     {
       KScope ks("pw_zfwd", st, 2);
-      k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p);
+      if (run64) k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, rl, 0);
+      if (run32)
+        k_pw_zfwd<NB, float2><<<gcol, 64, 0, st>>>(d, S32, d_group, d_active, loc_.p, rl, 1);
     }
     locv = loc_.p;
   };
@@ -754,19 +821,6 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     KScope ks("pw_plane", st, 2);
     k_pw_plane<<<dim3(n3_, nvec), kBlock, lds_plane, st>>>(d, in, in_stride, d_group, d_active,
                                                           S_.p);
-  }
-  SGW_HIP(hipGetLastError());
-  SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
-              "pw operator: prepare() was not called for this batch of vectors");
-  const int nchunk = (int)((ld + kChunk - 1) / kChunk);
-  if (nproj_ > 0) {
-    KScope ks("pw_becp", st, 2);
-    k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
-        d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
-        part_.p);
-    k_pw_becp_reduce<<<dim3((unsigned)((nproj_ * kVT + 63) / 64), ngroup_, plan_tiles_), 64, 0,
-                       st>>>(
-        d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk, part_.p, becp_.p);
   }
   SGW_HIP(hipGetLastError());
   {
@@ -796,7 +850,7 @@ void PwOperator::grad_accumulate(hipStream_t st, int np, const double2 *x, const
   d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
   d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
   d.ld = ld; d.nproj = nproj_;
-  d.vloc = vloc_.p; d.npw = d_npw_.p;
+  d.vloc = vloc_.p; d.vloc_zxy = vloc_zxy_.p; d.npw = d_npw_.p;
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;

@@ -72,27 +72,38 @@ __global__ __launch_bounds__(256) void k_sigma_f64(SigmaArgs a) {
     }
   const int arow = tid & 127, ak = tid >> 7;  // A staging: 8 (row, k) per thread
   const int bk = tid & 15, bc = tid >> 4;     // B staging: 4 (k, col) per thread
-  for (int j = 0; j < a.nin; ++j) {
+  // stages t = (j, kk): K runs over the G'' index of every convolution
+  // frequency in turn; the next stage's tiles are fetched into registers
+  // while the MFMAs of the current one run
+  const int nk = (n + kSBK - 1) / kSBK, nstage = a.nin * nk;
+  double2 ra[8], rb[4];
+  double rw = 0;
+  auto fetch = [&](int t) {
+    const int j = t / nk, kk = (t % nk) * kSBK;
     const double2 *Gij = a.G + (int64_t)i * a.gsi + (int64_t)j * a.gsj;
     const double2 *Wj = a.W + (int64_t)j * a.wsj;
-    const double wj = a.wt[j];
-    for (int kk = 0; kk < n; kk += kSBK) {
+    rw = a.wt[j];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int gm = m0 + arow, gk = kk + ak + 2 * r;
+      ra[r] = (gm < n && gk < n) ? Gij[gm + (int64_t)gk * ld] : mk(0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gk = kk + bk, gn = n0 + bc + 16 * r;
+      rb[r] = (gk < n && gn < n) ? Wj[gk + (int64_t)gn * ld] : mk(0, 0);
+    }
+  };
+  fetch(0);
+  for (int t = 0; t < nstage; ++t) {
+    {
       __syncthreads();
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int k = ak + 2 * r, gm = m0 + arow, gk = kk + k;
-        double2 v = mk(0, 0);
-        if (gm < n && gk < n) v = Gij[gm + (int64_t)gk * ld];
-        As[k][arow] = mk(v.x * wj, v.y * wj);
-      }
+      for (int r = 0; r < 8; ++r) As[ak + 2 * r][arow] = mk(ra[r].x * rw, ra[r].y * rw);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int c = bc + 16 * r, gk = kk + bk, gn = n0 + c;
-        double2 v = mk(0, 0);
-        if (gk < n && gn < n) v = Wj[gk + (int64_t)gn * ld];
-        Bs[bk][c] = v;
-      }
+      for (int r = 0; r < 4; ++r) Bs[bk][bc + 16 * r] = rb[r];
       __syncthreads();
+      if (t + 1 < nstage) fetch(t + 1);
 #pragma unroll
       for (int k4 = 0; k4 < kSBK / 4; ++k4) {
         const int kr = 4 * k4 + lk;
@@ -127,6 +138,121 @@ __global__ __launch_bounds__(256) void k_sigma_f64(SigmaArgs a) {
         if (row < n && col < n) {
           const double2 acc = mk(cre[r][c][q], cim[r][c][q]);
           double2 v = cmul(a.alpha, acc);
+          double2 *p = Si + row + (int64_t)col * ld;
+          if (a.accumulate) v = cadd(*p, v);
+          *p = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 split precision: the same tiling with operands rounded to FP32 when
+// staged and v_mfma_f32_16x16x4f32 products (twice the FP64 MFMA rate); the
+// FP32 accumulators hold the sum over one frequency j only (K = n) and are
+// flushed into FP64 accumulators before the next, so the rounding of the
+// 64-frequency sum stays FP64 and the error is that of an n-term FP32 dot.
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_sigma_f32s(SigmaArgs a) {
+  __shared__ float2 As[kSBK][kSBM + kSPad];
+  __shared__ float2 Bs[kSBK][kSBN + kSPad];
+  const int t = xcd_tile(blockIdx.x, a.nblocks);
+  const int tiles = a.tiles_m * a.tiles_n;
+  if (t >= tiles * a.nout) return;
+  const int i = t / tiles, tt = t % tiles;
+  const int m0 = (tt % a.tiles_m) * kSBM, n0 = (tt / a.tiles_m) * kSBN;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wm = (wv & 1) * 64, wn = (wv >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int n = a.n;
+  const int64_t ld = a.ld;
+  v4f fre[4][2], fim[4][2];   // FP32 partial sums of the current frequency
+  double dre[4][2][4], dim_[4][2][4];  // FP64 running sums
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      fre[r][c] = v4f{0, 0, 0, 0};
+      fim[r][c] = v4f{0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dre[r][c][q] = dim_[r][c][q] = 0.0;
+    }
+  const int arow = tid & 127, ak = tid >> 7;
+  const int bk = tid & 15, bc = tid >> 4;
+  const int nk = (n + kSBK - 1) / kSBK, nstage = a.nin * nk;
+  double2 ra[8], rb[4];
+  double rw = 0;
+  auto fetch = [&](int s) {
+    const int j = s / nk, kk = (s % nk) * kSBK;
+    const double2 *Gij = a.G + (int64_t)i * a.gsi + (int64_t)j * a.gsj;
+    const double2 *Wj = a.W + (int64_t)j * a.wsj;
+    rw = a.wt[j];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int gm = m0 + arow, gk = kk + ak + 2 * r;
+      ra[r] = (gm < n && gk < n) ? Gij[gm + (int64_t)gk * ld] : mk(0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gk = kk + bk, gn = n0 + bc + 16 * r;
+      rb[r] = (gk < n && gn < n) ? Wj[gk + (int64_t)gn * ld] : mk(0, 0);
+    }
+  };
+  fetch(0);
+  for (int s = 0; s < nstage; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+      As[ak + 2 * r][arow] = make_float2((float)(ra[r].x * rw), (float)(ra[r].y * rw));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Bs[bk][bc + 16 * r] = make_float2((float)rb[r].x, (float)rb[r].y);
+    __syncthreads();
+    if (s + 1 < nstage) fetch(s + 1);
+#pragma unroll
+    for (int k4 = 0; k4 < kSBK / 4; ++k4) {
+      const int kr = 4 * k4 + lk;
+      float2 av[4], bv[2];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) av[r] = As[kr][wm + 16 * r + lr];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) bv[c] = Bs[kr][wn + 16 * c + lr];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          fre[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r].x, bv[c].x, fre[r][c], 0, 0, 0);
+          fre[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(-av[r].y, bv[c].y, fre[r][c], 0, 0, 0);
+          fim[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r].x, bv[c].y, fim[r][c], 0, 0, 0);
+          fim[r][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[r].y, bv[c].x, fim[r][c], 0, 0, 0);
+        }
+    }
+    if ((s + 1) % nk == 0) {  // end of frequency j: flush the FP32 partials
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            dre[r][c][q] += (double)fre[r][c][q];
+            dim_[r][c][q] += (double)fim[r][c][q];
+          }
+          fre[r][c] = v4f{0, 0, 0, 0};
+          fim[r][c] = v4f{0, 0, 0, 0};
+        }
+    }
+  }
+  // v_mfma_f32_16x16x4f32 leaves D(4 (lane / 16) + q, lane % 16) in element q
+  double2 *Si = a.S + (int64_t)i * a.ssi;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = m0 + wm + 16 * r + 4 * lk + q, col = n0 + wn + 16 * c + lr;
+        if (row < n && col < n) {
+          double2 v = cmul(a.alpha, mk(dre[r][c][q], dim_[r][c][q]));
           double2 *p = Si + row + (int64_t)col * ld;
           if (a.accumulate) v = cadd(*p, v);
           *p = v;
@@ -214,7 +340,7 @@ int sgw_sigma_conv_device(const sgw_sigma_conv_desc *desc, const sgw_complex *d_
     const sgw_sigma_conv_desc &c = *desc;
     SGW_REQUIRE(c.n >= 1 && c.ld >= c.n && c.nout >= 1 && c.nin >= 1, SGW_E_SHAPE,
                 "sigma_conv: need n >= 1, ld >= n, nout >= 1, nin >= 1");
-    SGW_REQUIRE(c.precision == SGW_SIGMA_FP64, SGW_E_ARG,
+    SGW_REQUIRE(c.precision == SGW_SIGMA_FP64 || c.precision == SGW_SIGMA_FP32_SPLIT, SGW_E_ARG,
                 "sigma_conv: unsupported precision mode");
     const int64_t mat = c.ld * (int64_t)c.n;
     SGW_REQUIRE(c.w_stride >= mat && c.s_stride >= mat, SGW_E_SHAPE,
@@ -244,8 +370,13 @@ int sgw_sigma_conv_device(const sgw_sigma_conv_desc *desc, const sgw_complex *d_
     a.nblocks = (int)(((blocks + 7) / 8) * 8);
     hipStream_t st = (hipStream_t)stream;
     {
-      sgw::KScope ks("sigma_f64", st);
-      sgw::k_sigma_f64<<<a.nblocks, 256, 0, st>>>(a);
+      if (c.precision == SGW_SIGMA_FP64) {
+        sgw::KScope ks("sigma_f64", st);
+        sgw::k_sigma_f64<<<a.nblocks, 256, 0, st>>>(a);
+      } else {
+        sgw::KScope ks("sigma_f32s", st);
+        sgw::k_sigma_f32s<<<a.nblocks, 256, 0, st>>>(a);
+      }
     }
     SGW_HIP(hipGetLastError());
     return 0;

@@ -156,7 +156,7 @@ PROGRAM sgw_fixture_test
       sig2(:, 1) = sigma
       sig2(:, 2) = sigma
       grp = 0
-      bcfg = sgw_batch_config(2, num_shift, INT(vec_size, c_int64_t), 4, 10000, threshold, 0)
+      bcfg = sgw_batch_config(2, num_shift, INT(vec_size, c_int64_t), 4, 10000, threshold, 0, 0.0_dp)
       st = sgw_sternheimer_solve_batched(op, bcfg, grp, rhs2, sig2, x2, berr, bit)
       IF (st /= 0 .OR. ANY(berr /= 0)) nfail = nfail + 1
       worst = 0

@@ -66,6 +66,7 @@ MODULE sgw_capi
     INTEGER(c_int32_t) :: max_iter
     REAL(c_double)     :: threshold
     INTEGER(c_int32_t) :: shift_test   ! 0: seed residual only (reference); 1: every shift
+    REAL(c_double)     :: relax_threshold  ! 0: FP64; > 0: FP32 FFTs below this residual
   END TYPE sgw_batch_config
 
   INTERFACE

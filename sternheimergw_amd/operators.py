@@ -47,7 +47,7 @@ class Operator:
         return ax
 
     def solve_batched(self, rhs, sigma, group=None, lmax=4, threshold=1e-4, max_iter=10000,
-                      shift_test=False):
+                      shift_test=False, relax_threshold=0.0):
         """Batched multishift BiCGStab(l): rhs (nsys, n), sigma (nsys, nshift).
         shift_test=False stops a system on the reference's seed-residual test;
         True also requires every shifted residual estimate below threshold.
@@ -65,7 +65,8 @@ class Operator:
         x = np.zeros((nsys, nshift, ld), dtype=np.complex128)
         ierr = np.zeros(nsys, dtype=np.int32)
         iters = np.zeros(nsys, dtype=np.int32)
-        cfg = _lib.sgw_batch_config(nsys, nshift, ld, lmax, max_iter, threshold, int(shift_test))
+        cfg = _lib.sgw_batch_config(nsys, nshift, ld, lmax, max_iter, threshold, int(shift_test),
+                                    relax_threshold)
         _lib.check(_lib.lib().sgw_sternheimer_solve_batched(
             self.handle, ctypes.byref(cfg), _lib.ptr(g), _lib.ptr(rhs), _lib.ptr(sigma),
             _lib.ptr(x), _lib.ptr(ierr), _lib.ptr(iters)))

@@ -20,6 +20,7 @@ import numpy as np
 from . import _lib, cell
 
 SIGMA_FP64 = _lib.SIGMA_FP64
+SIGMA_FP32_SPLIT = _lib.SIGMA_FP32_SPLIT
 
 
 def g_vectors(n: int, alat: float = 10.26) -> np.ndarray:

@@ -99,7 +99,7 @@ def test_adjoint_finite_differences_full_size():
     L = SternheimerObjective.apply(v, prob)
     L.backward()
     grad = v.grad.numpy()
-    print(f"adjoint full: L = {float(L):.10e}", flush=True)
+    print(f"adjoint full: L = {L.item():.10e}", flush=True)
     rng = np.random.Generator(np.random.PCG64(13))
     h = 1e-5
     for t in range(2):

@@ -103,3 +103,28 @@ def test_full_size_sampled_entries():
     # the generated inputs themselves are the oracle's
     g0 = so.gen_g(n, i, 3, w.nout, w.nin, w.seed, w.nu[i, 3])[:, :8]
     assert np.allclose(d_g[0, 3, :8, :n].cpu().numpy().T, g0, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("n,ld,nout,nin", [(200, 200, 2, 5), (130, 131, 1, 3)])
+def test_conv_fp32_split_matches_oracle(n, ld, nout, nin):
+    """FP32 split precision: operands rounded to FP32, per-frequency sums in
+    FP32, the frequency sum in FP64.  Error bound 2e-5 relative (an n-term
+    FP32 dot of rounded operands, ~sqrt(n) 6e-8, with margin); it must also
+    be visibly above the FP64 kernel's, so the FP32 path is really taken."""
+    print(f"sigma_conv fp32 split n={n} nout={nout} nin={nin}", flush=True)
+    rng = np.random.Generator(np.random.PCG64(n + 3 * nin))
+    G, W = _rand(rng, nout, nin, n, n), _rand(rng, nin, n, n)
+    wt = rng.uniform(0.1, 1.0, nin)
+    alpha = complex(0.7, -0.2)
+    ref = so.sigma_conv(G, W, wt, alpha)
+    d_g, d_w = _to_dev_colmajor(G, ld), _to_dev_colmajor(W, ld)
+    d_wt = torch.from_numpy(wt).to(DEV)
+    d_s = _to_dev_colmajor(np.zeros((nout, n, n), dtype=complex), ld)
+    sigma.conv_device(d_g, d_w, d_wt, d_s, n, ld, nout, nin, alpha,
+                      precision=sigma.SIGMA_FP32_SPLIT)
+    torch.cuda.synchronize()
+    got = _from_dev_colmajor(d_s, n)
+    errs = [np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i]) for i in range(nout)]
+    print("fp32 split relative errors", errs, flush=True)
+    assert max(errs) <= 2e-5
+    assert min(errs) > 1e-12
