@@ -226,6 +226,19 @@ int sgw_sternheimer_adjoint(sgw_batch_solver *solver, const int32_t *d_group,
                             const sgw_complex *d_x, sgw_complex *d_y, double *d_grad,
                             double *h_objective, int32_t *h_ierr, void *stream);
 
+/* Density response of the solutions of one k+q group, the accumulation that
+ * follows the solve in solve_linter (phys/coul/src/solve_linter.f90:486-497,
+ * QE incdrhoscf per frequency):
+ *   drho[k](r) (+)= weight * sum_{v < nbnd} conj(psi_v(r)) dpsi[v][k](r),
+ * psi_v(r), dpsi(r) = invfft of plane-wave coefficients (QE sign, no factor).
+ * Device arrays: d_dpsi[nbnd][nshift][ld] in the order of `group`'s sphere
+ * (the batched solver's output for the systems of one q), d_psi_r[nbnd][box]
+ * and d_drho[nshift][box] with box index i1 + n1*(i2 + n2*i3).  QE's weight
+ * is 2 wk / Omega.  Cubic 24/32/36/48 boxes. */
+int sgw_drho_accumulate(sgw_operator *op, int32_t group, int32_t nbnd, int32_t nshift,
+                        const sgw_complex *d_dpsi, int64_t ld, const sgw_complex *d_psi_r,
+                        double weight, sgw_complex *d_drho, int32_t accumulate, void *stream);
+
 /* True relative residuals ||rhs[s] - (A + sigma[s][k]) x[s][k]|| / ||rhs[s]||
  * recomputed in FP64 on the device for every (system, shift); device arrays,
  * result d_res[nsys][nshift]. */

@@ -103,6 +103,8 @@ SIGNATURES = [
     ("sgw_sternheimer_adjoint", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                           c_void_p, c_void_p]),
+    ("sgw_drho_accumulate", c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_int64,
+                                      c_void_p, c_double, c_void_p, c_int32, c_void_p]),
     ("sgw_sigma_conv_device", c_int32, [ctypes.POINTER(sgw_sigma_conv_desc), c_void_p, c_void_p,
                                         c_void_p, sgw_complex, c_void_p, c_void_p]),
     ("sgw_sigma_synthetic_device", c_int32, [c_int32, c_int64, c_int32, c_int32, c_int32, c_int32,

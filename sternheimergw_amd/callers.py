@@ -107,3 +107,22 @@ def solve_linter_direct(op, dvpsi, group, eps, freq, wg, wk,
     else:                                                    # :472-479
         out[:, :, :] = 0.5 * out + 0.5 * x[:, nf:, :]
     return out
+
+
+def capture_linear_problem(op, group: int, rhs, sigma, x, path: str):
+    """Failure capture as in green.f90:415-446: the dense operator of k+q
+    group `group` is assembled column by column by applying the device
+    operator to unit vectors at sigma = 0 (all columns in one batched
+    application), and written with the shifts, the right-hand side and the
+    failed solution x (npw, nshift) as an iotk LINEAR_PROBLEM file that the
+    reference's solver test reads.  Returns the dense matrix."""
+    from . import iotk
+    n = int(op.npw[group]) if hasattr(op, "npw") else op.n
+    ld = op.ld
+    unit = np.zeros((n, ld), dtype=np.complex128)
+    unit[np.arange(n), np.arange(n)] = 1.0
+    cols = op.apply(unit, 0.0, group=np.full(n, group, dtype=np.int32))
+    hamil = np.ascontiguousarray(cols[:, :n].T)   # column j = H e_j
+    iotk.write_linear_problem(path, hamil, np.asarray(rhs)[:n], sigma,
+                              np.asarray(x).reshape(-1, np.size(sigma))[:n])
+    return hamil

@@ -194,6 +194,8 @@ class Workload:
     sigma: np.ndarray                # (nsys, nshift) complex shifts -(eps_v + omega)
     nbnd: int
     dense_h: Optional[List[np.ndarray]] = None  # per q dense H (small cells only)
+    sphere_k: Optional[Sphere] = None     # ground-state sphere at k = Gamma
+    psi_k: Optional[np.ndarray] = None    # (nbnd, npw_k) occupied states at k
     npert: int = 1                   # plane-wave perturbations e^{i(q+G)r} per (q, band)
     pert_q2: Optional[np.ndarray] = None  # (nsys,) |q+G|^2 (bohr^-2) of each system's perturbation
 
@@ -322,7 +324,8 @@ def build_workload(name: str, *, box, ecut_ha: float, nbnd: int, lchan: List[int
                     nproj_beta=nper * len(tau), proj=proj, dcoef=dcoef, alpha_pv=alpha_pv,
                     eps_v=eps_k, evq=evqs, rhs=rhs, group=group, omega=omega_ry,
                     sigma=sigma, nbnd=nbnd, dense_h=dense if keep_dense else None,
-                    npert=npert, pert_q2=pert_q2)
+                    sphere_k=sk, psi_k=np.ascontiguousarray(psi_k.T), npert=npert,
+                    pert_q2=pert_q2)
 
 
 def workload_c1(keep_dense=True, box=(24, 24, 24), **kw) -> Workload:

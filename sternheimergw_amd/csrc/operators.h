@@ -63,6 +63,12 @@ class PwOperator : public Operator {
   void grad_accumulate(hipStream_t st, int np, const double2 *x, const double2 *y,
                        int64_t stride, const int32_t *d_group, const double *d_w, int nb,
                        double *G, bool accumulate);
+  // Density response (incdrhoscf): D[k][z][x][y] (+)= wgt sum_v conj(psi_v(r))
+  // dpsi_{v,k}(r) for the nbnd x nshift vectors dpsi + (v nshift + k) stride
+  // of group `group`; Pr holds psi_v(r) as [v][z][x][y]; D has nshift boxes.
+  void drho_accumulate(hipStream_t st, int group, int nbnd, int nshift, const double2 *dpsi,
+                       int64_t stride, const double2 *Pr, double wgt, double2 *D,
+                       bool accumulate);
   // Precision of the local-potential FFT pipeline: 0 FP64, 1 FP32, 2 per
   // vector, FP32 where d_relax[v] != 0 (a device array of the caller, read at
   // every apply; the cubic FFT path only, else FP64).

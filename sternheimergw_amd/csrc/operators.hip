@@ -175,6 +175,28 @@ __device__ inline bool precision_skip(const int32_t *__restrict__ relax, int wan
   return relax != nullptr && relax[v] != want;
 }
 
+// dst[c] = src[c], c < n, by one wavefront: 16 loads per lane are issued
+// before the first LDS store, so a plane of up to 1024 column values costs
+// one memory latency instead of one per 64 values.
+template <typename C>
+__device__ inline void stage_plane(const C *__restrict__ src, C *dst, int n) {
+  constexpr int kB = 16;
+  const int lane = threadIdx.x & 63;
+  for (int c0 = 0; c0 < n; c0 += 64 * kB) {
+    C t[kB];
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+      const int c = c0 + b * 64 + lane;
+      t[b] = c < n ? src[c] : CplxT<C>::make(0, 0);
+    }
+#pragma unroll
+    for (int b = 0; b < kB; ++b) {
+      const int c = c0 + b * 64 + lane;
+      if (c < n) dst[c] = t[b];
+    }
+  }
+}
+
 // z inverse transform of the sphere columns: T[v][z][c] = sum_m3 x(c, m3)
 // e^{+2 pi i m3 z / N}.  One lane per column (m1, m2); the column's plane
 // waves are consecutive in sorted order with m3 = lo .. lo + cnt - 1 (the
@@ -277,7 +299,7 @@ __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const C *__restric
   C *P = T + nslot;     // nc1 * N
   const int32_t *cc = d.col_cnt + (int64_t)g * nslot;
   const C *Tz = Tin + ((int64_t)v * N * kSlotScale<C> + z) * nslot;
-  for (int c = lane; c < nslot; c += 64) T[c] = Tz[c];
+  stage_plane(Tz, T, nslot);
   __syncthreads();
   C buf[N];
   if (lane < nc1) {  // Y inverse of row a
@@ -338,7 +360,7 @@ __device__ inline void plane_to_real(const PwDev &d, const double2 *__restrict__
   const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M1 = d.M1, M2 = d.M2;
   const int lane = threadIdx.x;
   __syncthreads();
-  for (int c = lane; c < nslot; c += 64) T[c] = Tz[c];
+  stage_plane(Tz, T, nslot);
   __syncthreads();
   if (lane < nc1) {
 #pragma unroll
@@ -358,6 +380,43 @@ __device__ inline void plane_to_real(const PwDev &d, const double2 *__restrict__
       buf[i] = (m >= -M1 && m <= M1) ? P[(m + M1) * N + lane] : mk(0, 0);
     }
     reg_fft<N, +1>(buf);
+  }
+}
+
+// Density response of one z-plane for one shift k (incdrhoscf):
+//   D[k][z][x][y] (+)= wgt * sum_v conj(psi_v(r)) dpsi_{v,k}(r),
+// dpsi from the z-transformed columns T (vector v * nshift + k of the block)
+// and psi_v(r) given as Pr[v][z][x][y].  Block (z, k) owns its plane of D.
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_plane_drho(PwDev d, const double2 *__restrict__ T,
+                                                      int nbnd, int nshift,
+                                                      const double2 *__restrict__ Pr, double wgt,
+                                                      double2 *__restrict__ D, int accumulate) {
+  extern __shared__ double2 smem[];
+  const int z = blockIdx.x, k = blockIdx.y;
+  const int lane = threadIdx.x;
+  double2 *Ts = smem;              // nslot
+  double2 *P = Ts + d.nslot;       // nc1 * N
+  double2 *Dz = D + ((int64_t)k * N + z) * N * N;
+  if (!accumulate && lane < N)
+    for (int x = 0; x < N; ++x) Dz[(int64_t)x * N + lane] = mk(0, 0);
+  double2 buf[N];
+  for (int v = 0; v < nbnd; ++v) {
+    const int64_t vec = (int64_t)v * nshift + k;
+    plane_to_real<N>(d, T + (vec * N + z) * d.nslot, Ts, P, buf);
+    if (lane < N) {
+      const double2 *pz = Pr + ((int64_t)v * N + z) * N * N + lane;
+#pragma unroll
+      for (int x = 0; x < N; ++x) {
+        const double2 p = pz[(int64_t)x * N];
+        double2 *o = Dz + (int64_t)x * N + lane;
+        double2 acc = *o;
+        // conj(p) * dpsi
+        acc.x = fma(wgt, p.x * buf[x].x + p.y * buf[x].y, acc.x);
+        acc.y = fma(wgt, p.x * buf[x].y - p.y * buf[x].x, acc.y);
+        *o = acc;
+      }
+    }
   }
 }
 
@@ -831,6 +890,49 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
         plan_tiles_, nchunk, becp_.p, S_.p, locv);
   }
   SGW_HIP(hipGetLastError());
+}
+
+void PwOperator::drho_accumulate(hipStream_t st, int group, int nbnd, int nshift,
+                                 const double2 *dpsi, int64_t stride, const double2 *Pr,
+                                 double wgt, double2 *D, bool accumulate) {
+  const int nv = nbnd * nshift;
+  if (nv <= 0) return;
+  SGW_REQUIRE(group >= 0 && group < ngroup_, SGW_E_ARG, "pw operator: group index out of range");
+  const bool cubic = n1_ == n2_ && n2_ == n3_ && columns_contiguous_;
+  SGW_REQUIRE(cubic && (n1_ == 24 || n1_ == 32 || n1_ == 36 || n1_ == 48), SGW_E_ARG,
+              "pw operator: the density response needs a cubic 24/32/36/48 box");
+  const size_t lds = sizeof(double2) * (nslot_ + (size_t)nc1_ * n1_);
+  SGW_REQUIRE(nshift < 65536, SGW_E_ARG, "pw operator: too many shifts");
+  ensure_scratch(nv);
+  std::vector<int32_t> hg(nv, group);
+  DevBuf<int32_t> dg(nv);
+  dg.upload(hg.data(), nv, st);
+  PwDev d;
+  d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
+  d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
+  d.ld = ld; d.nproj = nproj_;
+  d.vloc = vloc_.p; d.vloc_zxy = vloc_zxy_.p; d.npw = d_npw_.p;
+  d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
+  d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
+  d.col_lo = col_lo_.p;
+  d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
+  d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
+  d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
+  const dim3 gcol((unsigned)((nslot_ + 63) / 64), nv);
+  auto run = [&](auto tag) {
+    constexpr int NB = decltype(tag)::value;
+    k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, dpsi, stride, dg.p, nullptr, T_.p);
+    k_pw_plane_drho<NB><<<dim3(NB, nshift), 64, lds, st>>>(d, T_.p, nbnd, nshift, Pr, wgt, D,
+                                                           accumulate ? 1 : 0);
+  };
+  switch (n1_) {
+    case 24: run(std::integral_constant<int, 24>{}); break;
+    case 32: run(std::integral_constant<int, 32>{}); break;
+    case 36: run(std::integral_constant<int, 36>{}); break;
+    default: run(std::integral_constant<int, 48>{}); break;
+  }
+  SGW_HIP(hipGetLastError());
+  SGW_HIP(hipStreamSynchronize(st));  // dg is released on return
 }
 
 void PwOperator::grad_accumulate(hipStream_t st, int np, const double2 *x, const double2 *y,

@@ -63,7 +63,14 @@ def decode(path=SRC):
 
 
 if __name__ == "__main__":
-    d = decode(sys.argv[1] if len(sys.argv) > 1 else SRC)
+    src = sys.argv[1] if len(sys.argv) > 1 else SRC
+    d = decode(src)
     np.savez_compressed(OUT, **{k: np.asarray(v) for k, v in d.items()})
     print("wrote", OUT, os.path.getsize(OUT), "bytes; n =", d["DIMENSION"],
           "shifts =", d["NUMBER_SHIFT"])
+    # digest of the decompressed stream: pins a writer of the format byte for byte
+    raw = bz2.open(src, "rb").read()
+    stream = os.path.join(os.path.dirname(OUT), "lin_prob_stream.json")
+    with open(stream, "w") as f:
+        f.write('{"bytes": %d, "sha256": "%s"}\n' % (len(raw), hashlib.sha256(raw).hexdigest()))
+    print("wrote", stream)
