@@ -69,7 +69,8 @@ __global__ __launch_bounds__(kBlock) void k_dotu(int nsys, const int32_t *__rest
   if (!active[s]) return;
   const double2 *pa = a + s * sa, *pb = b + s * sb;
   double2 acc = mk(0, 0);
-  for (int64_t i = threadIdx.x; i < n; i += kBlock) cmac(acc, pa[i], pb[i]);
+#pragma unroll 8
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) cmac(acc, pa[i], pb[i]);  // loads in flight
   acc = block_sum<kBlock>(acc, lds);
   if (threadIdx.x == 0) out[s] = acc;
 }
@@ -83,6 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_norm(int nsys, const int32_t *__rest
   if (!active[s]) return;
   const double2 *pa = a + s * sa;
   double2 acc = mk(0, 0);
+#pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += kBlock) {
     double2 v = pa[i];
     acc.x = fma(v.x, v.x, acc.x);

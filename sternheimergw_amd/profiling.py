@@ -20,6 +20,8 @@ import sys
 
 def _short(name: str) -> str:
     name = name.replace("(anonymous namespace)::", "").replace("void ", "")
+    name = name.replace("HIP_vector_type<double, 2u>", "double2").replace(
+        "HIP_vector_type<float, 2u>", "float2")
     depth, cut = 0, len(name)
     for i, ch in enumerate(name):  # drop the parameter list
         if ch == "<":
