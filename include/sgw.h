@@ -248,6 +248,82 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
                               double *d_res, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * The self-consistent response around the solves (solve_linter.f90:279-583).
+ * Boxes: cubic n^3, n in {24, 32, 36, 48}, [nvec][i3][i2][i1] (QE order
+ * i1 + n1*(i2 + n2*i3)); device arrays; asynchronous on `stream` unless
+ * stated.  `d_nl[i]` is the box index of plane wave i of a sphere (QE's
+ * dffts%nl(igk_k(i))).
+ * ------------------------------------------------------------------------- */
+
+/* QE fwfft/invfft of nvec boxes in place: sign +1 = invfft (sum e^{+iGr}, no
+ * factor), sign -1 = fwfft (sum e^{-iGr} / n^3).  FFTXlib (external). */
+int sgw_box_fft(int32_t n, int32_t nvec, int32_t sign, sgw_complex *d_box, void *stream);
+
+/* psi(r) = invfft of the sphere coefficients d_psi[nvec][ld] (i < npw)
+ * scattered through d_nl: QE cft_wave(+1) / the evc -> aux step of
+ * dvqpsi_us.f90:99-108 and solve_linter.f90:394. */
+int sgw_wave_to_box(int32_t n, int32_t nvec, int32_t npw, const int32_t *d_nl,
+                    const sgw_complex *d_psi, int64_t ld, sgw_complex *d_box, void *stream);
+
+/* psi[i] (+)= fwfft(box)[d_nl[i]] for i < npw (QE cft_wave(-1), which adds);
+ * d_box is overwritten by its transform; with accumulate = 0 the padding
+ * npw <= i < ld is zeroed. */
+int sgw_box_to_wave(int32_t n, int32_t nvec, int32_t npw, const int32_t *d_nl,
+                    sgw_complex *d_box, sgw_complex *d_psi, int64_t ld, int32_t accumulate,
+                    void *stream);
+
+/* dV psi on the k+q sphere for every frequency and band (dvqpsi_us.f90:99-132
+ * for the bare potential; solve_linter.f90:393-398 for dvscf):
+ *   dvpsi[f][v] = base[v] + gather_{k+q}( fwfft( psi_r[v](r) * dvscf[f](r) ) ),
+ * base[v] (sphere order of k+q, ld) may be NULL (then no base term).
+ * psi_r[nbnd][box] = invfft of the unperturbed states at k; dvscf[nfreq][box].
+ * Synchronises `stream` (scratch of at most 512 boxes). */
+int sgw_dvscf_psi(int32_t n, int32_t nbnd, int32_t nfreq, const sgw_complex *d_psi_r,
+                  const sgw_complex *d_dvscf, int32_t npwq, const int32_t *d_nlq,
+                  const sgw_complex *d_base, sgw_complex *d_dvpsi, int64_t ld, void *stream);
+
+/* QE orthogonalize (LR_Modules, called at solve_linter.f90:337, 411) for
+ * insulators with S = 1: dvpsi <- -(1 - |evq><evq|) dvpsi = -P_c^+ dvpsi for
+ * nvec vectors; evq[nocc][ld] the occupied states at k+q, nocc <= 64. */
+int sgw_orthogonalize(int32_t nvec, int32_t npw, int32_t nocc, const sgw_complex *d_evq,
+                      sgw_complex *d_dvpsi, int64_t ld, void *stream);
+
+/* The +/-omega average of solve_linter.f90:462-480 with the wg/wk rescale of
+ * :373 folded into `scale`: out[v][f] = scale * x(+omega_f) for f = 0 when
+ * zero_freq, else scale/2 (x(+omega_f) + x(-omega_f)), out[nbnd][nf][ld].
+ * layout 0: x[nbnd][nshift][ld] over the shift list of :237-252
+ * ([freq, -freq(2:)] when zero_freq, nshift = 2nf-1; else [freq, -freq]);
+ * layout 1: x[nbnd][nf][2][ld], shift 0 = +omega_f, shift 1 = -omega_f. */
+int sgw_average_pm(int32_t nbnd, int32_t nf, int32_t zero_freq, int32_t layout,
+                   const sgw_complex *d_x, int64_t ld, double scale, sgw_complex *d_out,
+                   void *stream);
+
+/* Hartree response potential of QE's dv_of_drho with lrpa (called at
+ * solve_linter.f90:556; lrpa set at main/src/gwq_readin.f90:223), in place:
+ *   dv(r) = invfft[ e2 4 pi / |q+G|^2 fwfft(drho)(G) ],  e2 = 2 (Ry),
+ * over |G|^2 <= gcut2 (QE's density sphere, bohr^-2), q + G = 0 dropped.
+ * zero_mean = 1 also drops G = 0 (the meandvb < eps10 branch,
+ * solve_linter.f90:545-549).  bg: b1, b2, b3 Cartesian (bohr^-1, 2 pi
+ * included), xq Cartesian (bohr^-1).  Miller index of box index i: i for
+ * i <= n/2, i - n above. */
+int sgw_dv_of_drho(int32_t n, int32_t nvec, const double *bg, const double *xq, double gcut2,
+                   int32_t zero_mean, sgw_complex *d_box, void *stream);
+
+/* Modified Broyden mixing of complex potentials, mix_potential_c
+ * (phys/coul/src/mix_pot_c.f90:25-190; D. D. Johnson, PRB 38, 12807), same
+ * arguments: on return vout = vout - vin, vin = the mixed potential,
+ * *dr2 = (||vout - vin|| / ndim)^2, *conv = dr2 < tr2 (vin unchanged then).
+ * The history of earlier iterations (the reference's SAVE arrays) lives in
+ * the sgw_broyden state: n_iter <= 8 stored iterations of ndim elements.
+ * Synchronises `stream`. */
+typedef struct sgw_broyden sgw_broyden;
+sgw_broyden *sgw_broyden_create(int64_t ndim, int32_t n_iter);
+void sgw_broyden_destroy(sgw_broyden *b);
+int sgw_mix_potential_c(sgw_broyden *b, int64_t ndim, sgw_complex *d_vout, sgw_complex *d_vin,
+                        double alphamix, double *dr2, double tr2, int32_t iter, int32_t n_iter,
+                        int32_t *conv, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Sigma_c frequency convolution (phys/corr/src/sigma.f90:705-745, the
  * accumulation sigma(:,:,isigma) += alpha * w * [G (x) W] over the Green's
  * function frequencies), restated as a batched complex GEMM on the device:

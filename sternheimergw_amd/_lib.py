@@ -105,6 +105,24 @@ SIGNATURES = [
                                           c_void_p, c_void_p]),
     ("sgw_drho_accumulate", c_int32, [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_int64,
                                       c_void_p, c_double, c_void_p, c_int32, c_void_p]),
+    ("sgw_box_fft", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p]),
+    ("sgw_wave_to_box", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int64,
+                                  c_void_p, c_void_p]),
+    ("sgw_box_to_wave", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+                                  c_int64, c_int32, c_void_p]),
+    ("sgw_dvscf_psi", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int32,
+                                c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    ("sgw_orthogonalize", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_int64,
+                                    c_void_p]),
+    ("sgw_dv_of_drho", c_int32, [c_int32, c_int32, c_void_p, c_void_p, c_double, c_int32,
+                                 c_void_p, c_void_p]),
+    ("sgw_average_pm", c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_int64, c_double,
+                                 c_void_p, c_void_p]),
+    ("sgw_broyden_create", c_void_p, [c_int64, c_int32]),
+    ("sgw_broyden_destroy", None, [c_void_p]),
+    ("sgw_mix_potential_c", c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_double,
+                                      ctypes.POINTER(c_double), c_double, c_int32, c_int32,
+                                      ctypes.POINTER(c_int32), c_void_p]),
     ("sgw_sigma_conv_device", c_int32, [ctypes.POINTER(sgw_sigma_conv_desc), c_void_p, c_void_p,
                                         c_void_p, sgw_complex, c_void_p, c_void_p]),
     ("sgw_sigma_synthetic_device", c_int32, [c_int32, c_int64, c_int32, c_int32, c_int32, c_int32,
