@@ -249,7 +249,7 @@ int sgw_residual_norms_device(sgw_operator *op, int32_t nsys, int32_t nshift, in
 
 /* ---------------------------------------------------------------------------
  * The self-consistent response around the solves (solve_linter.f90:279-583).
- * Boxes: cubic n^3, n in {24, 32, 36, 48}, [nvec][i3][i2][i1] (QE order
+ * Boxes: cubic n^3, n in {16, 18, 24, 27, 32, 36, 48}, [nvec][i3][i2][i1] (QE order
  * i1 + n1*(i2 + n2*i3)); device arrays; asynchronous on `stream` unless
  * stated.  `d_nl[i]` is the box index of plane wave i of a sphere (QE's
  * dffts%nl(igk_k(i))).
@@ -322,6 +322,41 @@ void sgw_broyden_destroy(sgw_broyden *b);
 int sgw_mix_potential_c(sgw_broyden *b, int64_t ndim, sgw_complex *d_vout, sgw_complex *d_vin,
                         double alphamix, double *dr2, double tr2, int32_t iter, int32_t n_iter,
                         int32_t *conv, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Green's function of the self-energy: assembly (green.f90:196-218) and the
+ * 6-D Fourier transforms (data/fft/src/fft6.f90).  A two-point function is a
+ * Fortran matrix f(nr, nr_p) per frequency (first index fastest), nr = n^3,
+ * nr_p = n_p^3 (the correlation boxes corr_fft and corr_par_fft, cubic, n in
+ * {16, 18, 24, 27, 32, 36, 48}); in reciprocal space only the (num_g,
+ * num_g_p) top-left block is defined.  d_nl / d_nl_p: box index of each G
+ * (QE's dfft%nl, 0-based).  nmat matrices back to back; synchronises.
+ * ------------------------------------------------------------------------- */
+
+/* invfft6 (fft6.f90:215-345):
+ *   f(r, r') = 1/omega sum_{G,G'} e^{-iGr} f(G, G') e^{+iG'r'}. */
+int sgw_invfft6(int32_t n, int32_t n_p, int32_t num_g, const int32_t *d_nl, int32_t num_g_p,
+                const int32_t *d_nl_p, double omega, int32_t nmat, sgw_complex *d_f,
+                void *stream);
+
+/* fwfft6 (fft6.f90:84-181):
+ *   f(G, G') = omega / (nr nr_p) sum_{r,r'} e^{+iGr} f(r, r') e^{-iG'r'}. */
+int sgw_fwfft6(int32_t n, int32_t n_p, int32_t num_g, const int32_t *d_nl, int32_t num_g_p,
+               const int32_t *d_nl_p, double omega, int32_t nmat, sgw_complex *d_f,
+               void *stream);
+
+/* green(nr, nr_p, nfreq) from the batched solutions of green_function's
+ * columns (green.f90:184-186, 196-218):
+ *   green(r, c, f) = x[colsys[c]][f][rowmap[r]]  for r < nrow, c < ncol with
+ *   both maps >= 0, and zero elsewhere.
+ * rowmap[r]: 0-based position of correlation G r in the k-point's sphere, -1
+ * when the reference's mask (0 < map < num_g, :212) excludes it; colsys[c]:
+ * the batched system that solved column c (-1 when green.f90:200 skips it).
+ * x[nsys][nshift][ld] with shift f = frequency f (nfreq <= nshift). */
+int sgw_green_assemble(int32_t nfreq, int32_t n, int32_t n_p, int32_t nrow,
+                       const int32_t *d_rowmap, int32_t ncol, const int32_t *d_colsys,
+                       const sgw_complex *d_x, int32_t nshift, int64_t ld, sgw_complex *d_green,
+                       void *stream);
 
 /* ---------------------------------------------------------------------------
  * Sigma_c frequency convolution (phys/corr/src/sigma.f90:705-745, the

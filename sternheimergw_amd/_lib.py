@@ -118,6 +118,12 @@ SIGNATURES = [
                                  c_void_p, c_void_p]),
     ("sgw_average_pm", c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_int64, c_double,
                                  c_void_p, c_void_p]),
+    ("sgw_invfft6", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_double,
+                              c_int32, c_void_p, c_void_p]),
+    ("sgw_fwfft6", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_double,
+                             c_int32, c_void_p, c_void_p]),
+    ("sgw_green_assemble", c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32,
+                                     c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
     ("sgw_broyden_create", c_void_p, [c_int64, c_int32]),
     ("sgw_broyden_destroy", None, [c_void_p]),
     ("sgw_mix_potential_c", c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_double,

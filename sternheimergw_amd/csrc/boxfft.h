@@ -1,0 +1,17 @@
+// Whole-box 3-D FFTs of cubic n^3 boxes in QE's index order
+// (i1 + n*(i2 + n*i3), i1 fastest), nvec boxes back to back, in place.
+#pragma once
+
+#include "sgw_internal.h"
+
+namespace sgw {
+
+// Box edges with a register FFT (2^a 3^b): 16, 18, 24, 27, 32, 36, 48.
+bool box_supported(int n);
+const char *box_sizes_text();
+
+// SIGN = +1: sum e^{+iGr} (QE invfft, no factor); SIGN = -1: sum e^{-iGr}
+// without the 1/n^3 of QE's fwfft.  Throws for unsupported n.
+void box_fft(hipStream_t st, int n, int64_t nvec, double2 *d, int sign);
+
+}  // namespace sgw

@@ -11,17 +11,16 @@
 //    (solve_linter.f90:556; main/src/gwq_readin.f90:223 sets lrpa);
 //  * the modified Broyden mixing of mix_potential_c (phys/coul/src/mix_pot_c.f90).
 //
-// Boxes are cubic n^3 (n in {24, 32, 36, 48}), stored [vector][i3][i2][i1]
-// with i1 fastest (QE's i1 + n1*(i2 + n2*i3)).  The box FFT is two passes:
-// a 2-D (x, y) transform of each z-plane staged through LDS, then one
-// register-resident z line per thread (coalesced across x).
+// Boxes are cubic n^3 (n in {16, 18, 24, 27, 32, 36, 48}), stored
+// [vector][i3][i2][i1] with i1 fastest (QE's i1 + n1*(i2 + n2*i3)); the box
+// FFTs are those of boxfft.hip.
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
 
+#include "boxfft.h"
 #include "ktimer.h"
-#include "regfft.h"
 #include "sgw_internal.h"
 
 namespace sgw {
@@ -31,69 +30,9 @@ constexpr int kEB = 256;       // elementwise kernels
 constexpr int kRB = 256;       // reduction kernels
 constexpr int kRedBlocks = 512;
 
-// ---------------------------------------------------------------------------
-// box FFT
-// ---------------------------------------------------------------------------
-template <int N, int SIGN>
-__global__ __launch_bounds__(64) void k_box_xy(double2 *__restrict__ d) {
-  constexpr int P = N + 1;  // padded LDS row
-  __shared__ double2 s[N * P];
-  double2 *g = d + (int64_t)blockIdx.x * N * N;
-  for (int e = threadIdx.x; e < N * N; e += 64) s[(e / N) * P + e % N] = g[e];
-  __syncthreads();
-  const int t = threadIdx.x;
-  if (t < N) {  // x line y = t
-    double2 x[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = s[t * P + i];
-    reg_fft<N, SIGN>(x);
-#pragma unroll
-    for (int i = 0; i < N; ++i) s[t * P + i] = x[i];
-  }
-  __syncthreads();
-  if (t < N) {  // y line x = t
-    double2 x[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) x[i] = s[i * P + t];
-    reg_fft<N, SIGN>(x);
-#pragma unroll
-    for (int i = 0; i < N; ++i) s[i * P + t] = x[i];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < N * N; e += 64) g[e] = s[(e / N) * P + e % N];
-}
-
-template <int N, int SIGN>
-__global__ __launch_bounds__(64) void k_box_z(int64_t nvec, double2 *__restrict__ d) {
-  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (t >= nvec * N * N) return;
-  const int64_t v = t / (N * N), c = t % (N * N);
-  double2 *p = d + v * N * N * N + c;
-  double2 x[N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) x[i] = p[(int64_t)i * N * N];
-  reg_fft<N, SIGN>(x);
-#pragma unroll
-  for (int i = 0; i < N; ++i) p[(int64_t)i * N * N] = x[i];
-}
-
-template <int N, int SIGN> void box_fft_t(hipStream_t st, int64_t nvec, double2 *d) {
-  if (nvec <= 0) return;
-  k_box_xy<N, SIGN><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
-  k_box_z<N, SIGN><<<(unsigned)((nvec * N * N + 63) / 64), 64, 0, st>>>(nvec, d);
-}
-
-bool box_supported(int n) { return n == 24 || n == 32 || n == 36 || n == 48; }
-
 // SIGN = +1: QE invfft (no factor); SIGN = -1: unscaled fwfft.
 template <int SIGN> void box_fft(hipStream_t st, int n, int64_t nvec, double2 *d) {
-  switch (n) {
-    case 24: box_fft_t<24, SIGN>(st, nvec, d); break;
-    case 32: box_fft_t<32, SIGN>(st, nvec, d); break;
-    case 36: box_fft_t<36, SIGN>(st, nvec, d); break;
-    case 48: box_fft_t<48, SIGN>(st, nvec, d); break;
-    default: throw Error{SGW_E_SHAPE, "box FFT: cubic boxes of 24, 32, 36 or 48 points only"};
-  }
+  ::sgw::box_fft(st, n, nvec, d, SIGN);
 }
 
 // ---------------------------------------------------------------------------
@@ -466,7 +405,7 @@ extern "C" int sgw_box_fft(int32_t n, int32_t nvec, int32_t sign, sgw_complex *d
                            void *stream) {
   return guarded([&] {
     SGW_REQUIRE(d_box && nvec >= 0, SGW_E_ARG, "box_fft: null box or negative count");
-    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "box_fft: n must be 24, 32, 36 or 48");
+    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "box_fft: n must be 16, 18, 24, 27, 32, 36 or 48");
     SGW_REQUIRE(sign == 1 || sign == -1, SGW_E_ARG, "box_fft: sign must be +1 or -1");
     hipStream_t st = (hipStream_t)stream;
     double2 *d = reinterpret_cast<double2 *>(d_box);
@@ -489,7 +428,7 @@ extern "C" int sgw_wave_to_box(int32_t n, int32_t nvec, int32_t npw, const int32
                                void *stream) {
   return guarded([&] {
     SGW_REQUIRE(d_nl && d_psi && d_box && nvec >= 0, SGW_E_ARG, "wave_to_box: null argument");
-    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "wave_to_box: n must be 24, 32, 36 or 48");
+    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "wave_to_box: n must be 16, 18, 24, 27, 32, 36 or 48");
     SGW_REQUIRE(npw >= 0 && npw <= ld, SGW_E_SHAPE, "wave_to_box: need 0 <= npw <= ld");
     hipStream_t st = (hipStream_t)stream;
     const int64_t nbox = (int64_t)n * n * n;
@@ -512,7 +451,7 @@ extern "C" int sgw_box_to_wave(int32_t n, int32_t nvec, int32_t npw, const int32
                                int32_t accumulate, void *stream) {
   return guarded([&] {
     SGW_REQUIRE(d_nl && d_psi && d_box && nvec >= 0, SGW_E_ARG, "box_to_wave: null argument");
-    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "box_to_wave: n must be 24, 32, 36 or 48");
+    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "box_to_wave: n must be 16, 18, 24, 27, 32, 36 or 48");
     SGW_REQUIRE(npw >= 0 && npw <= ld, SGW_E_SHAPE, "box_to_wave: need 0 <= npw <= ld");
     hipStream_t st = (hipStream_t)stream;
     const int64_t nbox = (int64_t)n * n * n;
@@ -535,7 +474,7 @@ extern "C" int sgw_dvscf_psi(int32_t n, int32_t nbnd, int32_t nfreq, const sgw_c
                              void *stream) {
   return guarded([&] {
     SGW_REQUIRE(d_psi_r && d_dvscf && d_nlq && d_dvpsi, SGW_E_ARG, "dvscf_psi: null argument");
-    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "dvscf_psi: n must be 24, 32, 36 or 48");
+    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "dvscf_psi: n must be 16, 18, 24, 27, 32, 36 or 48");
     SGW_REQUIRE(nbnd >= 1 && nfreq >= 1 && npwq >= 1 && npwq <= ld, SGW_E_SHAPE,
                 "dvscf_psi: need nbnd, nfreq >= 1 and 1 <= npwq <= ld");
     hipStream_t st = (hipStream_t)stream;
@@ -605,7 +544,7 @@ extern "C" int sgw_dv_of_drho(int32_t n, int32_t nvec, const double *bg, const d
                               double gcut2, int32_t zero_mean, sgw_complex *d_box, void *stream) {
   return guarded([&] {
     SGW_REQUIRE(bg && xq && d_box && nvec >= 0, SGW_E_ARG, "dv_of_drho: null argument");
-    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "dv_of_drho: n must be 24, 32, 36 or 48");
+    SGW_REQUIRE(box_supported(n), SGW_E_SHAPE, "dv_of_drho: n must be 16, 18, 24, 27, 32, 36 or 48");
     hipStream_t st = (hipStream_t)stream;
     const int64_t nbox = (int64_t)n * n * n;
     double2 *d = reinterpret_cast<double2 *>(d_box);

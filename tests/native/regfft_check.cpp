@@ -48,7 +48,9 @@ template <int N> void both(unsigned seed) {
 
 int main() {
   both<16>(1);
+  both<18>(3);
   both<24>(5);
+  both<27>(7);
   both<32>(9);
   both<36>(13);
   both<48>(17);
