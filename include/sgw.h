@@ -359,6 +359,41 @@ int sgw_green_assemble(int32_t nfreq, int32_t n, int32_t n_p, int32_t nrow,
                        void *stream);
 
 /* ---------------------------------------------------------------------------
+ * The reference form of Sigma_c: analytic continuation of W
+ * (algo/analytic/src/analytic.f90) and the real-space product sigma_prod
+ * (phys/corr/src/sigma.f90:417-523).  W arrays are Fortran
+ * scrcoul(num_g, num_g, nfreq) (first index fastest), device memory.
+ * ------------------------------------------------------------------------- */
+#define SGW_GODBY_NEEDS 1   /* analytic.f90:33, godby_needs.f90 */
+#define SGW_PADE_APPROX 2   /* analytic.f90:36, pade.f90 (Vidberg-Serene) */
+
+/* analytic_coeff (analytic.f90:50-208) in place: W on the frequency mesh
+ * becomes the model's coefficients.  Godby-Needs: nfreq = 2, h_z[1] = i
+ * omega_p (freq%solver(2)).  Pade: the mesh of freqbins_symm, h_z[nfreq]
+ * (nfreq <= 64), and h_src[i] = the slice of d_w that holds W(h_z[i]) (the
+ * even-symmetry copy of freqbins_symm; NULL = i). */
+int sgw_analytic_coeff(int32_t model, int32_t num_g, int32_t nfreq, const sgw_complex *h_z,
+                       const int32_t *h_src, sgw_complex *d_w, void *stream);
+
+/* analytic_eval (analytic.f90:211-348): out(nr, nr_p) = 0, then
+ * out(r, c) = W_model(freq; coeff(gmap[r], gmap_p[c], :)) for r < num_g,
+ * c < num_g_p (gmap: gmapsym, gmap_p: gmapsym(fft_map), 0-based).  freq is
+ * the symmetrised output frequency (freq_in%symmetrize). */
+int sgw_analytic_eval(int32_t model, int32_t num_gc, int32_t nfreq, const sgw_complex *h_z,
+                      const sgw_complex *d_coeff, sgw_complex freq, int32_t n, int32_t n_p,
+                      int32_t num_g, int32_t num_g_p, const int32_t *d_gmap,
+                      const int32_t *d_gmap_p, sgw_complex *d_out, void *stream);
+
+/* sigma_prod (sigma.f90:417-523) on one W: d_array (nr, nr_p) holds W(G, G')
+ * in its (num_g, num_g_p) block on input and alpha fwfft6[G(r,r') W(r,r')]
+ * there on output; d_green is G(r, r') (nr, nr_p).  With d_sigma (ld_sigma >=
+ * num_g) the block is also added to Sigma (sigma.f90:741).  Synchronises. */
+int sgw_sigma_prod(int32_t n, int32_t n_p, int32_t num_g, const int32_t *d_nl, int32_t num_g_p,
+                   const int32_t *d_nl_p, double omega, sgw_complex alpha,
+                   const sgw_complex *d_green, sgw_complex *d_array, sgw_complex *d_sigma,
+                   int64_t ld_sigma, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Sigma_c frequency convolution (phys/corr/src/sigma.f90:705-745, the
  * accumulation sigma(:,:,isigma) += alpha * w * [G (x) W] over the Green's
  * function frequencies), restated as a batched complex GEMM on the device:

@@ -71,6 +71,8 @@ class sgw_sigma_conv_desc(ctypes.Structure):
 
 SIGMA_FP64 = 0
 SIGMA_FP32_SPLIT = 1
+GODBY_NEEDS = 1
+PADE_APPROX = 2
 
 # (name, restype, argtypes) of every exported entry point of include/sgw.h
 SIGNATURES = [
@@ -124,6 +126,13 @@ SIGNATURES = [
                              c_int32, c_void_p, c_void_p]),
     ("sgw_green_assemble", c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32,
                                      c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
+    ("sgw_analytic_coeff", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
+                                     c_void_p]),
+    ("sgw_analytic_eval", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, sgw_complex,
+                                    c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p,
+                                    c_void_p, c_void_p]),
+    ("sgw_sigma_prod", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_double,
+                                 sgw_complex, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     ("sgw_broyden_create", c_void_p, [c_int64, c_int32]),
     ("sgw_broyden_destroy", None, [c_void_p]),
     ("sgw_mix_potential_c", c_int32, [c_void_p, c_int64, c_void_p, c_void_p, c_double,

@@ -1,0 +1,23 @@
+// 6-D Fourier transforms of two-point functions (fft6.f90), callable from
+// other translation units of the library.
+#pragma once
+
+#include "sgw_internal.h"
+
+namespace sgw {
+
+struct Fft6Args {
+  int n, n_p, num_g, num_g_p;
+  const int32_t *nl, *nl_p;  // device box-index maps of the two G lists
+  double omega;              // cell volume
+  int nmat;
+  double2 *f;                // nmat matrices f(nr, nr_p), Fortran layout
+};
+
+void fft6_check(const Fft6Args &a, const char *who);
+
+// One matrix at a time on `st`; `scratch` holds num_g * n_p^3 elements.
+void invfft6_run(hipStream_t st, const Fft6Args &a, double2 *scratch);
+void fwfft6_run(hipStream_t st, const Fft6Args &a, double2 *scratch);
+
+}  // namespace sgw

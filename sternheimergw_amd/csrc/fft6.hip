@@ -23,6 +23,7 @@
 #include <algorithm>
 
 #include "boxfft.h"
+#include "fft6.h"
 #include "ktimer.h"
 #include "sgw_internal.h"
 
@@ -97,25 +98,6 @@ __global__ void k_green_assemble(int nfreq, int64_t nr, int64_t nr_p, int nrow, 
   green[e] = v;
 }
 
-struct Fft6Args {
-  int n, n_p, num_g, num_g_p;
-  const int32_t *nl, *nl_p;
-  double omega;
-  int nmat;
-  double2 *f;
-};
-
-void check_args(const Fft6Args &a, const char *who) {
-  SGW_REQUIRE(a.f && a.nl && a.nl_p && a.nmat >= 0, SGW_E_ARG,
-              std::string(who) + ": null argument or negative count");
-  SGW_REQUIRE(box_supported(a.n) && box_supported(a.n_p), SGW_E_SHAPE,
-              std::string(who) + ": box edges must be " + box_sizes_text());
-  const int64_t nr = (int64_t)a.n * a.n * a.n, nr_p = (int64_t)a.n_p * a.n_p * a.n_p;
-  SGW_REQUIRE(a.num_g >= 1 && a.num_g <= nr && a.num_g_p >= 1 && a.num_g_p <= nr_p, SGW_E_SHAPE,
-              std::string(who) + ": need 1 <= num_g <= nr and 1 <= num_g_p <= nr_p");
-  SGW_REQUIRE(a.omega > 0.0, SGW_E_ARG, std::string(who) + ": the cell volume must be positive");
-}
-
 template <typename F> int guarded(F &&fn) {
   try {
     return fn();
@@ -129,6 +111,47 @@ template <typename F> int guarded(F &&fn) {
 }
 
 }  // namespace
+
+void fft6_check(const Fft6Args &a, const char *who) {
+  SGW_REQUIRE(a.f && a.nl && a.nl_p && a.nmat >= 0, SGW_E_ARG,
+              std::string(who) + ": null argument or negative count");
+  SGW_REQUIRE(box_supported(a.n) && box_supported(a.n_p), SGW_E_SHAPE,
+              std::string(who) + ": box edges must be " + box_sizes_text());
+  const int64_t nr = (int64_t)a.n * a.n * a.n, nr_p = (int64_t)a.n_p * a.n_p * a.n_p;
+  SGW_REQUIRE(a.num_g >= 1 && a.num_g <= nr && a.num_g_p >= 1 && a.num_g_p <= nr_p, SGW_E_SHAPE,
+              std::string(who) + ": need 1 <= num_g <= nr and 1 <= num_g_p <= nr_p");
+  SGW_REQUIRE(a.omega > 0.0, SGW_E_ARG, std::string(who) + ": the cell volume must be positive");
+}
+
+void invfft6_run(hipStream_t st, const Fft6Args &a, double2 *s) {
+  const int n = a.n, n_p = a.n_p, num_g = a.num_g, num_g_p = a.num_g_p;
+  const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
+  for (int m = 0; m < a.nmat; ++m) {
+    double2 *f = a.f + (int64_t)m * nr * nr_p;
+    k_fill0<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>((int64_t)num_g * nr_p, s);
+    k_rows_scatter<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
+        num_g, num_g_p, nr, nr_p, a.nl_p, f, 1.0 / a.omega, s);
+    box_fft(st, n_p, num_g, s, +1);
+    k_fill0<<<blocks(nr * nr_p), kEB, 0, st>>>(nr * nr_p, f);
+    k_cols_scatter<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, a.nl, s, f);
+    box_fft(st, n, nr_p, f, -1);
+  }
+}
+
+void fwfft6_run(hipStream_t st, const Fft6Args &a, double2 *s) {
+  const int n = a.n, n_p = a.n_p, num_g = a.num_g, num_g_p = a.num_g_p;
+  const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
+  for (int m = 0; m < a.nmat; ++m) {
+    double2 *f = a.f + (int64_t)m * nr * nr_p;
+    box_fft(st, n, nr_p, f, +1);
+    k_cols_gather<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, a.nl, f,
+                                                                  1.0 / (double)nr, s);
+    box_fft(st, n_p, num_g, s, -1);
+    k_rows_gather<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
+        num_g, num_g_p, nr, nr_p, a.nl_p, s, a.omega / (double)nr_p, f);
+  }
+}
+
 }  // namespace sgw
 
 using namespace sgw;
@@ -139,22 +162,11 @@ extern "C" int sgw_invfft6(int32_t n, int32_t n_p, int32_t num_g, const int32_t 
   return guarded([&] {
     Fft6Args a{n, n_p, num_g, num_g_p, d_nl, d_nl_p, omega, nmat,
                reinterpret_cast<double2 *>(d_f)};
-    check_args(a, "invfft6");
+    fft6_check(a, "invfft6");
     hipStream_t st = (hipStream_t)stream;
-    const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
-    DevBuf<double2> S((size_t)num_g * nr_p);  // one matrix at a time on the stream
+    DevBuf<double2> S((size_t)num_g * n_p * n_p * n_p);  // one matrix at a time
     KScope ks("invfft6", st);
-    for (int m = 0; m < nmat; ++m) {
-      double2 *f = a.f + (int64_t)m * nr * nr_p;
-      double2 *s = S.p;
-      k_fill0<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>((int64_t)num_g * nr_p, s);
-      k_rows_scatter<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
-          num_g, num_g_p, nr, nr_p, d_nl_p, f, 1.0 / omega, s);
-      box_fft(st, n_p, num_g, s, +1);
-      k_fill0<<<blocks(nr * nr_p), kEB, 0, st>>>(nr * nr_p, f);
-      k_cols_scatter<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, d_nl, s, f);
-      box_fft(st, n, nr_p, f, -1);
-    }
+    invfft6_run(st, a, S.p);
     SGW_HIP(hipGetLastError());
     SGW_HIP(hipStreamSynchronize(st));  // scratch freed on return
     return 0;
@@ -167,21 +179,11 @@ extern "C" int sgw_fwfft6(int32_t n, int32_t n_p, int32_t num_g, const int32_t *
   return guarded([&] {
     Fft6Args a{n, n_p, num_g, num_g_p, d_nl, d_nl_p, omega, nmat,
                reinterpret_cast<double2 *>(d_f)};
-    check_args(a, "fwfft6");
+    fft6_check(a, "fwfft6");
     hipStream_t st = (hipStream_t)stream;
-    const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
-    DevBuf<double2> S((size_t)num_g * nr_p);  // one matrix at a time on the stream
+    DevBuf<double2> S((size_t)num_g * n_p * n_p * n_p);
     KScope ks("fwfft6", st);
-    for (int m = 0; m < nmat; ++m) {
-      double2 *f = a.f + (int64_t)m * nr * nr_p;
-      double2 *s = S.p;
-      box_fft(st, n, nr_p, f, +1);
-      k_cols_gather<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, d_nl, f,
-                                                                    1.0 / (double)nr, s);
-      box_fft(st, n_p, num_g, s, -1);
-      k_rows_gather<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
-          num_g, num_g_p, nr, nr_p, d_nl_p, s, omega / (double)nr_p, f);
-    }
+    fwfft6_run(st, a, S.p);
     SGW_HIP(hipGetLastError());
     SGW_HIP(hipStreamSynchronize(st));
     return 0;
