@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -59,9 +61,12 @@ def parse():
                     help="after the timed steps, one untimed step with per-kernel events")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (MiB) for counter calibration")
-    ap.add_argument("--workload", choices=("solve", "sigma"), default="solve",
+    ap.add_argument("--workload", choices=("solve", "sigma", "response", "sigma_real"),
+                    default="solve",
                     help="solve: the batched Sternheimer solve (headline); sigma: the Sigma_c "
-                         "frequency convolution GEMM (N_G = 1500, 64 x 64 frequencies)")
+                         "frequency convolution GEMM (N_G = 1500, 64 x 64 frequencies); "
+                         "response: one dielectric column (direct solve_linter, 48^3); "
+                         "sigma_real: the reference-form Sigma_c products (24^3, N_G 1500)")
     ap.add_argument("--sigma-precision", choices=("fp64", "fp32s"), default="fp64",
                     help="fp32s: FP32 MFMA products, per-frequency FP32 sums, FP64 frequency sum")
     ap.add_argument("--sigma-batch", type=int, default=8,
@@ -160,6 +165,189 @@ def run_sigma(args):
     }))
 
 
+def _gl_freq_ry(nf, wmax_ha=4.0):
+    """[0] + (nf - 1) Gauss-Legendre imaginary frequencies on [0, wmax] (Ry)."""
+    from sternheimergw_amd import cell
+    x, _w = cell.gauss_legendre(0.0, wmax_ha, nf - 1)
+    return np.concatenate([[0.0], 1j * x * cell.HA2RY])
+
+
+def run_response(args):
+    """Columns of the dielectric matrix (coulomb.f90:115-170 around the direct
+    solve_linter) on one k+q group of the 48^3 cell for the 16 shortest
+    nonzero G: dvbare psi, -P_c, one batched multishift solve of 16 G x 16
+    bands over [freq, -freq(2:)] (16 frequencies, 31 shifts), the +/-omega
+    average, incdrhoscf, dv_of_drho and the gather at 100 G'.  A step is the
+    16 columns."""
+    import torch
+
+    from oracle import oracle as orc
+    from oracle import scf_oracle as sco
+    from sternheimergw_amd import _lib, cell, operators, scf
+
+    L = _lib.lib()
+    t_setup = time.perf_counter()
+    w = cell.workload_c2(nq=1, nbnd=16)
+    op = operators.PwOperator.from_workload(w)
+    setup = scf.LinterSetup(op, w, 0)
+    _a, bg, omega_cell = cell.fcc_lattice(w.alat)
+    freq = _gl_freq_ry(16)
+    gcut2 = 2.0 * 4.0 * 25.0
+    g = np.array([(i, j, k) for i in range(-3, 4) for j in range(-3, 4) for k in range(-3, 4)])
+    g = g[np.argsort((g @ bg).__pow__(2).sum(1), kind="stable")][:100]
+    npert = 16
+    pms = g[1:1 + npert]
+    pm = pms[0]
+    setup_s = time.perf_counter() - t_setup
+    stats = {}
+
+    def step():
+        return scf.coulomb_columns_direct(setup, pms, freq, g, bg=bg, gcut2=gcut2,
+                                          threshold=args.threshold, lmax=args.lmax, stats=stats)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(L.sgw_kernel_timing(1))
+    stats.clear()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        cols = step()
+        col = cols[0]
+        print(f"step {s}: eps_GG(q, 0) = {col[0, 1]:.6f}", flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms, cnt = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(L.sgw_kernel_timing_query(b"happly", ctypes.byref(ms), ctypes.byref(cnt)))
+    _lib.check(L.sgw_kernel_timing(0))
+    nbox = int(np.prod(w.box))
+    npw = float(w.npw[0])
+    bh = algorithmic_bytes_per_happly(nbox, npw, w.nproj, w.nbnd)
+    vec = stats.get("vector_happlies", 0)
+    achieved = bh * vec / (ms.value / 1e3) / 1e9 if ms.value > 0 else 0.0
+    out = {
+        "metric": "dielectric-matrix columns/s (direct solve_linter + coulomb, 48^3 FFT, "
+                  "16 bands, 31 shifts, 1e-8)",
+        "value": round(npert * args.steps / elapsed, 4),
+        "unit": "columns/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (seeded fcc cell, V_loc, beta projectors; FP64 diagonalisation)",
+        "config": {"workload": "response-48^3: 16 eps columns per step (16 G x 16 bands in one "
+                               "batch), 16 frequencies (31 shifts), G' = 100",
+                   "setup_s": round(setup_s, 2), "eps_00": [col[0, 1].real, col[0, 1].imag]},
+        "roofline": {"kernel": "H-apply (pw_zinv+pw_plane+pw_zfwd+pw_becp+pw_assemble)",
+                     "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "vector_happlies": vec, "launches": cnt.value,
+                     "avg_launch_ms": round(ms.value / max(cnt.value, 1), 4)},
+    }
+    if args.cpu_baseline:
+        # the oracle's band solves dominate its solve_linter: time two bands at
+        # the same threshold and scale to 16
+        n = w.box[0]
+        nlk, nlq = w.sphere_k.box_index(w.box), w.spheres[0].box_index(w.box)
+        dvb = sco.invfft(np.eye(1, n ** 3, int((pm[0] % n) + n * ((pm[1] % n) + n * (pm[2] % n))),
+                                dtype=np.complex128).reshape(n, n, n))
+        psi_r = sco.wave_to_box(n, nlk, w.psi_k[:2])
+        rhs = sco.orthogonalize(sco.dvscf_psi(n, psi_r, dvb[None], nlq)[0], w.evq[0][:, :])
+        omegas, _zf = sco.coulomb_frequencies(freq)
+        o = orc.PwOp(w, 0)
+        tc = time.perf_counter()
+        for v in range(2):
+            orc.bicgstab(o, rhs[v], -(w.eps_v[v] + omegas), lmax=args.lmax,
+                         threshold=args.threshold)
+        tc = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": round(1.0 / (tc / 2 * w.nbnd), 6), "unit": "columns/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"2 of 16 band solves (31 shifts) with the oracle on one "
+                                         f"core, {tc:.1f} s, scaled to 16 bands"}
+    print(json.dumps(out))
+
+
+def run_sigma_real(args):
+    """The reference form of Sigma_c (sigma.f90:653-745) on a 24^3 correlation
+    box with N_G = 1500: W continued by Pade from 4 imaginary frequencies, 4
+    Coulomb frequencies (8 Green's-function frequencies) and 2 Sigma
+    frequencies, 16 sigma_prods per step (invfft6 of W, G W, fwfft6)."""
+    import torch
+
+    from oracle import green_oracle as go
+    from oracle import sigma_real_oracle as sro
+    from sternheimergw_amd import _lib, cell, green, sigma_real
+
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    n, ng = 24, 1500
+    gv = cell.shortest_millers(ng)
+    grid = green.CorrGrid(n, gv)
+    omega_cell = 270.011
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    ngreen, nsig = 8, 2
+    d_green = torch.randn((ngreen, n ** 3, n ** 3), dtype=torch.complex128, device=dev,
+                          generator=gen) * 1e-3
+    solver = 1j * np.array([0.0, 0.5, 1.5, 4.0])
+    d_w = torch.randn((solver.size, ng, ng), dtype=torch.complex128, device=dev, generator=gen)
+    scr = sigma_real.ScreenedCoulomb(d_w, solver, _lib.PADE_APPROX, sigma_real.NO_SYMMETRY)
+    coul = 1j * np.array([0.1, 0.6, 1.5, 3.0])
+    weight = np.array([0.2, 0.3, 0.3, 0.2])
+    sig_freq = np.array([-0.2, 0.3])
+
+    def step():
+        return sigma_real.sigma_correlation(grid, grid, omega_cell, 0.5, 0.05, coul, weight,
+                                            sig_freq, d_green, scr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.check(L.sgw_kernel_timing(1))
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        print(f"step {s}: {time.perf_counter() - t0:.2f} s", flush=True)
+    elapsed = time.perf_counter() - t0
+    ms, cnt = ctypes.c_double(), ctypes.c_int64()
+    _lib.check(L.sgw_kernel_timing_query(b"sigma_prod", ctypes.byref(ms), ctypes.byref(cnt)))
+    _lib.check(L.sgw_kernel_timing(0))
+    nr = n ** 3
+    bytes_per = 16.0 * (4 * ng * nr + 7 * nr * nr)
+    t_prod = ms.value / 1e3 / max(cnt.value, 1)
+    achieved = bytes_per / t_prod / 1e9
+    prods = ngreen * nsig * args.steps
+    out = {
+        "metric": "Sigma_c real-space products/s (sigma_prod: invfft6 W, G*W, fwfft6; 24^3 box, "
+                  "N_G 1500)",
+        "value": round(prods / elapsed, 3), "unit": "sigma_prod/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (random G(r,r') and W(G,G',i w) on the device)",
+        "config": {"workload": "sigma-real-24^3: N_G 1500, Pade W (4 points), 8 Green x 2 Sigma "
+                               "frequencies per step", "n": n, "num_g": ng},
+        "roofline": {"kernel": "sigma_prod (box FFT passes + index maps + product)",
+                     "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per, "launches": cnt.value,
+                     "avg_launch_ms": round(t_prod * 1e3, 3)},
+    }
+    if args.cpu_baseline:
+        rng = np.random.default_rng(3)
+        G = (rng.standard_normal((nr, nr)) + 1j * rng.standard_normal((nr, nr))) * 1e-3
+        W = np.zeros((nr, nr), dtype=np.complex128)
+        W[:ng, :ng] = rng.standard_normal((ng, ng))
+        tc = time.perf_counter()
+        sro.sigma_prod(n, n, go.box_index(gv, n), go.box_index(gv, n), omega_cell, 0.5, G, W)
+        tc = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": round(1.0 / tc, 5), "unit": "sigma_prod/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"one oracle sigma_prod (fft6 on the oracle FFT) at the "
+                                         f"same size on one core, {tc:.1f} s"}
+    print(json.dumps(out))
+
+
 def algorithmic_bytes_per_happly(nbox, npw, nproj, nvec_per_group, sc=16, sr=8):
     """B_H per H-apply per vector: a 3-pass fused FFT pipeline touching the box
     4 times (complex) + V(r) once + psi in/out + projectors amortised over the
@@ -241,7 +429,12 @@ def main():
     if args.workload == "sigma":
         run_sigma(args)
         return
-    import numpy as np
+    if args.workload == "response":
+        run_response(args)
+        return
+    if args.workload == "sigma_real":
+        run_sigma_real(args)
+        return
     import torch
 
     from sternheimergw_amd import _lib, cell, operators

@@ -180,8 +180,9 @@ class LinterSetup:
         self.xq = sq.kvec - w.sphere_k.kvec
 
 
-def _solve(setup, d_rhs, sigma, threshold, lmax, max_iter, shift_test):
-    """Batched multishift solve of rhs (nsys, ld) at sigma (nsys, nshift)."""
+def _solve(setup, d_rhs, sigma, threshold, lmax, max_iter, shift_test, stats=None):
+    """Batched multishift solve of rhs (nsys, ld) at sigma (nsys, nshift);
+    adds the vectors the operator was applied to into stats."""
     import torch
     nsys, nshift = sigma.shape
     dev = d_rhs.device
@@ -190,19 +191,22 @@ def _solve(setup, d_rhs, sigma, threshold, lmax, max_iter, shift_test):
     d_group = torch.full((nsys,), setup.group, dtype=torch.int32, device=dev)
     d_sigma = torch.from_numpy(np.ascontiguousarray(sigma)).to(dev)
     d_x = torch.zeros((nsys, nshift, setup.ld), dtype=torch.complex128, device=dev)
-    ierr, _iters = bs.solve(d_group, d_rhs, d_sigma, sigma, d_x)
+    ierr, iters = bs.solve(d_group, d_rhs, d_sigma, sigma, d_x)
     if np.any(ierr != 0):
         raise SolverError("solver did not converge")  # errore, solve_linter.f90:370
+    if stats is not None:
+        stats["vector_happlies"] = stats.get("vector_happlies", 0) + int(2 * lmax * iters.sum())
     return d_x
 
 
 def solve_linter(setup: LinterSetup, dvbare, freq, num_iter: int, *, bg, gcut2: float,
                  threshold: float = 1e-10, lmax: int = 4, alpha_mix=0.7, nmix: int = 4,
                  tr2: float = 1e-14, wk: float = 2.0, volume: float = None,
-                 max_iter: int = 10000, pair_pm: bool = True, shift_test: bool = True):
+                 max_iter: int = 10000, pair_pm: bool = True, shift_test: bool = True,
+                 stats: dict = None):
     """dvbare: (n, n, n) complex, the bare potential on the box (a host array
     or a device tensor).  Returns (drhoscf (nf, n, n, n) device tensor,
-    iterations)."""
+    iterations).  stats, when given, accumulates "vector_happlies"."""
     import torch
     w, n, ld, nbnd = setup.w, setup.n, setup.ld, setup.nbnd
     dev = setup.d_psi_r.device
@@ -233,7 +237,7 @@ def solve_linter(setup: LinterSetup, dvbare, freq, num_iter: int, *, bg, gcut2: 
             orthogonalize(rhs, setup.d_evq, setup.npwq, ld)
             thresh = threshold if direct else 1e-2                      # :347-360
             sigma = -(eps[:, None] + omegas[None, :])                   # :369
-            x = _solve(setup, rhs, sigma, thresh, lmax, max_iter, shift_test)
+            x = _solve(setup, rhs, sigma, thresh, lmax, max_iter, shift_test, stats)
             dpsi = average_pm(x, nbnd, nf, zero_freq, 0, ld)
         else:
             thresh = min(0.1 * math.sqrt(dr2), 1e-2)                     # :429-430
@@ -249,7 +253,7 @@ def solve_linter(setup: LinterSetup, dvbare, freq, num_iter: int, *, bg, gcut2: 
                 sig[:, :, 0] = -(eps[:, None] + omegas[None, :nf])
                 sig[:, :, 1] = -(eps[:, None] + om_m[None, :])
                 x = _solve(setup, rhs.reshape(nbnd * nf, ld), sig.reshape(nbnd * nf, 2),
-                           thresh, lmax, max_iter, shift_test)
+                           thresh, lmax, max_iter, shift_test, stats)
                 dpsi = average_pm(x, nbnd, nf, zero_freq, 1, ld)
             else:
                 # the reference's single-shift solves, systems (v, k) over the shift list
@@ -258,7 +262,7 @@ def solve_linter(setup: LinterSetup, dvbare, freq, num_iter: int, *, bg, gcut2: 
                 rhs = torch.stack([rhs_f[kf[k]] for k in range(ns)], dim=1).contiguous()  # (nbnd, ns, ld)
                 sig = -(eps[:, None] + omegas[None, :])
                 x = _solve(setup, rhs.reshape(nbnd * ns, ld), sig.reshape(nbnd * ns, 1),
-                           thresh, lmax, max_iter, shift_test)
+                           thresh, lmax, max_iter, shift_test, stats)
                 dpsi = average_pm(x, nbnd, nf, zero_freq, 0, ld)
         # incdrhoscf for every frequency; the direct result is -dv_of_drho(drho),
         # so the sign goes into the weight there
@@ -302,3 +306,54 @@ def coulomb_column(setup: LinterSetup, pert_mill, freq, num_iter: int, g_mill, *
         diag = np.all(gm == pm[None, :], axis=1)
         res[:, diag] += 1.0
     return res, it
+
+
+def coulomb_columns_direct(setup: LinterSetup, pert_mills, freq, g_mill, *, bg, gcut2: float,
+                           threshold: float = 1e-10, lmax: int = 4, wk: float = 2.0,
+                           volume: float = None, max_iter: int = 10000, shift_test: bool = True,
+                           stats: dict = None):
+    """The direct solver's columns of coulomb.f90:115-170 for every G in
+    pert_mills at once: the reference runs one solve_linter per G
+    (coulomb.f90:118-138); with the direct solver the G's do not couple, so
+    all (G, band) systems form one batched multishift solve and every other
+    step runs over all G's in one call.  Returns eps[ipert, iw, igp] (host),
+    the same numbers coulomb_column gives one G at a time."""
+    import torch
+    w, n, ld, nbnd = setup.w, setup.n, setup.ld, setup.nbnd
+    dev = setup.d_psi_r.device
+    pms = np.asarray(pert_mills, dtype=np.int64).reshape(-1, 3)
+    npert = pms.shape[0]
+    gm = np.asarray(g_mill, dtype=np.int64).reshape(-1, 3)
+    bgm = np.asarray(bg, dtype=np.float64).reshape(3, 3)
+    omega_cell = volume if volume is not None else abs(np.linalg.det(2 * np.pi * np.linalg.inv(bgm).T))
+    weight = 2.0 * wk / omega_cell
+    freq = np.asarray(freq, dtype=np.complex128)
+    nf = freq.size
+    omegas, zero_freq = coulomb_frequencies(freq)
+    # dvbare of every G (coulomb.f90:130-135); |dvbare| = 1, so meandvb > eps10
+    d_dvbare = torch.zeros((npert, n ** 3), dtype=torch.complex128, device=dev)
+    idx = torch.from_numpy((pms[:, 0] % n) + n * ((pms[:, 1] % n) + n * (pms[:, 2] % n))).to(dev)
+    d_dvbare[torch.arange(npert, device=dev), idx] = 1.0
+    box_fft(d_dvbare.view(npert, n, n, n), +1)
+    # dvbare psi_v for every (G, v), then -P_c^+ (solve_linter.f90:330-337)
+    rhs = dvscf_psi(setup.d_psi_r, d_dvbare.view(npert, n, n, n), setup.d_nlq, None, ld)
+    rhs = rhs.reshape(npert * nbnd, ld)
+    orthogonalize(rhs, setup.d_evq, setup.npwq, ld)
+    eps = np.tile(np.asarray(w.eps_v, dtype=np.float64), npert)
+    sigma = -(eps[:, None] + omegas[None, :])                        # :369
+    x = _solve(setup, rhs, sigma, threshold, lmax, max_iter, shift_test, stats)
+    dpsi = average_pm(x, npert * nbnd, nf, zero_freq, 0, ld)         # (npert*nbnd, nf, ld)
+    drho = torch.empty((npert, nf, n, n, n), dtype=torch.complex128, device=dev)
+    for p in range(npert):                                           # incdrhoscf, sign of :596
+        _lib.check(_lib.lib().sgw_drho_accumulate(
+            setup.op.handle, setup.group, nbnd, nf, _p(dpsi[p * nbnd:(p + 1) * nbnd]), ld,
+            _p(setup.d_psi_r), -weight, _p(drho[p]), 0, _stream()))
+    dv_of_drho(drho.view(npert * nf, n, n, n), bg, setup.xq, gcut2, zero_mean=False)
+    nl = ((gm[:, 0] % n) + n * ((gm[:, 1] % n) + n * (gm[:, 2] % n))).astype(np.int32)
+    d_nl = torch.from_numpy(nl).to(dev)
+    out = torch.empty((npert * nf, nl.size), dtype=torch.complex128, device=dev)
+    box_to_wave(drho.view(npert * nf, n, n, n), d_nl, out, nl.size)
+    res = out.cpu().numpy().reshape(npert, nf, nl.size)
+    for p in range(npert):                                           # coulomb.f90:155-159
+        res[p][:, np.all(gm == pms[p][None, :], axis=1)] += 1.0
+    return res

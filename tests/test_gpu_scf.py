@@ -221,3 +221,18 @@ def test_coulomb_column_direct(linter):
     assert np.abs(col - ref).max() <= 1e-8 * np.abs(ref).max()
     # eps_GG' is 1 - v chi0 on the diagonal: real part above 1 at omega = 0
     assert col[0, 1].real > 1.0
+
+
+def test_coulomb_columns_batched_equal_one_at_a_time(linter):
+    w, _op, setup, b, omega = linter
+    freq = np.array([0.0, 1.0j])
+    gcut2 = 2 * 4 * 6.0
+    pms = np.array([[1, 0, 0], [0, 1, 0], [-1, 1, 1]])
+    g_mill = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [-1, 1, 1], [0, 0, 1]])
+    print("coulomb columns: 3 perturbations in one batch", flush=True)
+    eps = scf.coulomb_columns_direct(setup, pms, freq, g_mill, bg=b, gcut2=gcut2,
+                                     threshold=1e-11)
+    for p, pm in enumerate(pms):
+        col, _it = scf.coulomb_column(setup, pm, freq, 1, g_mill, bg=b, gcut2=gcut2,
+                                      threshold=1e-11)
+        assert np.abs(eps[p] - col).max() <= 1e-8 * np.abs(col).max(), p
