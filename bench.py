@@ -38,7 +38,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "happly_traffic.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--threshold", type=float, default=1e-8)
     ap.add_argument("--lmax", type=int, default=4)
@@ -48,6 +48,10 @@ def parse():
     ap.add_argument("--shift-test", type=int, default=1,
                     help="1: a system stops when every shift's residual estimate passes the "
                          "threshold; 0: the reference's test on the seed residual alone")
+    ap.add_argument("--stream", type=int, default=1,
+                    help="1: the timed steps' batches pass through the solver as one queue "
+                         "(sgw_batch_solver_run_stream), slots refilled as systems finish; "
+                         "0: one lock-step batched solve per step")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="mixed: FP32 local-potential FFTs once a system's residual estimate "
                          "is below --relax-threshold (vectors, recurrences and test in FP64)")
@@ -471,19 +475,62 @@ def main():
                                     ctypes.c_void_p(iters.ctypes.data), ctypes.c_void_p(stream))
         _lib.check(st)
 
-    for i in range(args.warmup):
-        step()
-        print(f"warmup {i}: max iterations {iters.max()}, ierr max {ierr.max()}", flush=True)
+    # streaming: the steps' batches as one queue through the nsys slots
+    # (each batch is the same 8192 solves; every solve is completed and
+    # checked below)
+    nq = nsys * max(args.steps, args.warmup, 1)
+    if args.stream and relax > 0:
+        raise SystemExit("bench.py: --stream runs FP64 only (use --stream 0 with --precision mixed)")
+    if args.stream:
+        reps = nq // nsys
+        d_rhs_q = d_rhs.repeat(reps, 1).contiguous()
+        d_sigma_q = d_sigma.repeat(reps, 1).contiguous()
+        h_sigma_q = np.ascontiguousarray(np.tile(h_sigma, (reps, 1)))
+        h_group_q = np.ascontiguousarray(np.tile(w.group, reps), dtype=np.int32)
+        d_x_q = torch.zeros((nq, nshift, ld), dtype=torch.complex128, device=dev)
+        ierr_q = np.zeros(nq, dtype=np.int32)
+        iters_q = np.zeros(nq, dtype=np.int32)
+
+    def stream_run(nbatch):
+        n = nbatch * nsys
+        st = L.sgw_batch_solver_run_stream(solver, n, ctypes.c_void_p(h_group_q.ctypes.data),
+                                           ctypes.c_void_p(d_rhs_q.data_ptr()),
+                                           ctypes.c_void_p(d_sigma_q.data_ptr()),
+                                           ctypes.c_void_p(h_sigma_q.ctypes.data),
+                                           ctypes.c_void_p(d_x_q.data_ptr()),
+                                           ctypes.c_void_p(ierr_q.ctypes.data),
+                                           ctypes.c_void_p(iters_q.ctypes.data),
+                                           ctypes.c_void_p(stream))
+        _lib.check(st)
+
+    if args.stream:
+        if args.warmup:
+            stream_run(args.warmup)
+            print(f"warmup: {args.warmup} batches streamed, max iterations "
+                  f"{iters_q[:args.warmup * nsys].max()}", flush=True)
+    else:
+        for i in range(args.warmup):
+            step()
+            print(f"warmup {i}: max iterations {iters.max()}, ierr max {ierr.max()}", flush=True)
     torch.cuda.synchronize()
     _lib.check(L.sgw_kernel_timing(args.kernel_timing))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     total_vec_happly = 0
-    for i in range(args.steps):
-        step()
-        total_vec_happly += int(2 * args.lmax * iters.sum())
-        print(f"step {i}: iterations min/median/max {iters.min()}/{int(np.median(iters))}/"
-              f"{iters.max()}", flush=True)
+    if args.stream:
+        stream_run(args.steps)
+        its = iters_q[:args.steps * nsys]
+        total_vec_happly = int(2 * args.lmax * its.sum())
+        iters[:] = its[:nsys]
+        ierr[:] = ierr_q[:nsys]
+        print(f"streamed {args.steps} batches: iterations min/median/max {its.min()}/"
+              f"{int(np.median(its))}/{its.max()}", flush=True)
+    else:
+        for i in range(args.steps):
+            step()
+            total_vec_happly += int(2 * args.lmax * iters.sum())
+            print(f"step {i}: iterations min/median/max {iters.min()}/{int(np.median(iters))}/"
+                  f"{iters.max()}", flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
@@ -505,7 +552,10 @@ def main():
         # per-kernel split from one extra, untimed step (events around every launch)
         _lib.check(L.sgw_kernel_timing(2))
         tb = time.perf_counter()
-        step()
+        if args.stream:
+            stream_run(1)
+        else:
+            step()
         torch.cuda.synchronize()
         tb = time.perf_counter() - tb
         per = query(happly_kernels + solver_kernels)
@@ -514,18 +564,24 @@ def main():
                          "avg_us": round(v[0] * 1e3 / max(v[1], 1), 2), "launches": v[1]}
                      for k, v in per.items()}
 
-    # true residuals of every (system, shift) in FP64
+    # true residuals of every (system, shift) in FP64: the last step's batch
+    # (lock-step), or every streamed batch of the timed run
     d_res = torch.zeros((nsys, nshift), dtype=torch.float64, device=dev)
-    _lib.check(L.sgw_residual_norms_device(op.handle, nsys, nshift, ld,
-                                           ctypes.c_void_p(d_group.data_ptr()),
-                                           ctypes.c_void_p(d_rhs.data_ptr()),
-                                           ctypes.c_void_p(d_sigma.data_ptr()),
-                                           ctypes.c_void_p(d_x.data_ptr()),
-                                           ctypes.c_void_p(d_res.data_ptr()),
-                                           ctypes.c_void_p(stream)))
-    res = d_res.cpu().numpy()
-    good = int((res <= RES_TARGET).sum())
-    solves_per_s = good * args.steps / elapsed
+    batches = ([d_x_q[b * nsys:(b + 1) * nsys] for b in range(args.steps)] if args.stream
+               else [d_x])
+    good, res_max = 0, 0.0
+    for xb in batches:
+        _lib.check(L.sgw_residual_norms_device(op.handle, nsys, nshift, ld,
+                                               ctypes.c_void_p(d_group.data_ptr()),
+                                               ctypes.c_void_p(d_rhs.data_ptr()),
+                                               ctypes.c_void_p(d_sigma.data_ptr()),
+                                               ctypes.c_void_p(xb.data_ptr()),
+                                               ctypes.c_void_p(d_res.data_ptr()),
+                                               ctypes.c_void_p(stream)))
+        res = d_res.cpu().numpy()
+        good += int((res <= RES_TARGET).sum())
+        res_max = max(res_max, float(res.max()))
+    solves_per_s = good * (1 if args.stream else args.steps) / elapsed
 
     # roofline of the H-apply (one event pair around its kernels on the solver stream)
     nbox = int(np.prod(w.box))
@@ -544,7 +600,8 @@ def main():
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
         if prof.get("workload") == [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold,
-                                    args.shift_test, relax]:
+                                    args.shift_test, relax, args.stream,
+                                    args.steps if args.stream else 1]:
             traffic = prof["traffic_bytes_per_happly"]
             traffic_note = {"source": "profiles/happly_traffic.json",
                             "algorithmic_bytes_per_launch": round(bh * total_vec_happly
@@ -573,8 +630,11 @@ def main():
             "threshold": args.threshold, "lmax": args.lmax, "shift_test": args.shift_test,
             "relax_threshold": relax,
             "solves_per_step": nsys * nshift, "solves_meeting_1e-8": good,
-            "max_true_residual": float(res.max()),
+            "solves_checked": len(batches) * nsys * nshift,
+            "max_true_residual": res_max,
             "iterations_max": int(iters.max()), "iterations_median": float(np.median(iters)),
+            "execution": (f"streamed: the {args.steps} batches as one queue of {nq} systems "
+                          f"through {nsys} slots" if args.stream else "lock-step per batch"),
             "setup_s": round(setup_s, 2),
         },
         "roofline": {

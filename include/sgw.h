@@ -200,6 +200,22 @@ int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
                          const sgw_complex *h_sigma, sgw_complex *d_x, int32_t *h_ierr,
                          int32_t *h_iters, void *stream);
 
+/* A queue of nq systems through the solver's nsys slots (the callers' loops
+ * over bands, q and G' of solve_linter.f90:366-374 and green.f90:196-218
+ * as one stream).  Each operator group keeps a fixed share of the slots, and
+ * a slot whose system has finished takes the next queued system of its
+ * group, so the batch stays full until the queue runs dry.  Every system
+ * runs exactly the iterations of sgw_batch_solver_run (a system's iterates do
+ * not depend on the other slots).  h_group[nq] host; d_rhs[nq][ld],
+ * d_sigma/h_sigma[nq][nshift], d_x[nq][nshift][ld]; h_ierr/h_iters[nq].
+ * Device-resident operators; relax_threshold is ignored (FP64).  A system
+ * that reaches max_iter stops one iteration later than in the lock-step
+ * solve (ierr 1 either way). */
+int sgw_batch_solver_run_stream(sgw_batch_solver *solver, int32_t nq, const int32_t *h_group,
+                                const sgw_complex *d_rhs, const sgw_complex *d_sigma,
+                                const sgw_complex *h_sigma, sgw_complex *d_x, int32_t *h_ierr,
+                                int32_t *h_iters, void *stream);
+
 /* One-shot form on host arrays (copies in and out). */
 int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
                                   const int32_t *group, const sgw_complex *rhs,

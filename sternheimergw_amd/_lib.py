@@ -97,6 +97,8 @@ SIGNATURES = [
     ("sgw_batch_solver_destroy", None, [c_void_p]),
     ("sgw_batch_solver_run", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p]),
+    ("sgw_batch_solver_run_stream", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     ("sgw_sternheimer_solve_batched", c_int32, [c_void_p, ctypes.POINTER(sgw_batch_config),
                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                                 c_void_p]),

@@ -52,6 +52,21 @@ class BatchSolver:
             self._stream()))
         return ierr, iters
 
+    def solve_stream(self, h_group, d_rhs, d_sigma, h_sigma, d_x):
+        """A queue of len(h_group) systems through the solver's nsys slots
+        (sgw_batch_solver_run_stream); d_rhs (nq, ld), d_sigma (nq, nshift),
+        d_x (nq, nshift, ld).  Returns (ierr, iters) per queued system."""
+        h_group = np.ascontiguousarray(h_group, dtype=np.int32)
+        nq = h_group.size
+        ierr = np.zeros(max(nq, 1), dtype=np.int32)
+        iters = np.zeros(max(nq, 1), dtype=np.int32)
+        h_sigma = np.ascontiguousarray(h_sigma, dtype=np.complex128)
+        _lib.check(_lib.lib().sgw_batch_solver_run_stream(
+            self.handle, nq, ctypes.c_void_p(h_group.ctypes.data), _p(d_rhs), _p(d_sigma),
+            ctypes.c_void_p(h_sigma.ctypes.data), _p(d_x), ctypes.c_void_p(ierr.ctypes.data),
+            ctypes.c_void_p(iters.ctypes.data), self._stream()))
+        return ierr[:nq], iters[:nq]
+
     def adjoint(self, d_group, d_c, d_sigma, h_sigma, d_weight, d_x, d_y, d_grad=None):
         """Backward solve at conj(sigma) into d_y and, with d_grad (a real
         tensor of the operator's box, (n3, n2, n1)), dL/dV_loc for

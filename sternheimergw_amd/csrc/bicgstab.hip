@@ -119,9 +119,16 @@ __global__ void k_bicg_begin(int nsys, const int32_t *__restrict__ active, Scal 
 // `commit` first completes L18 of the previous step (alpha_old = alpha =
 // alpha_new), which k_bicg_s2 leaves to this kernel so that its per-shift
 // threads can all read the old alpha.
-__global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ active, Scal sc) {
+// `fresh` (streaming solves, first step of an iteration only): a system in its
+// first iteration has no previous step to complete; the flag is consumed here.
+__global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ active, Scal sc,
+                          int32_t *__restrict__ fresh = nullptr) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys || !active[s]) return;
+  if (fresh) {
+    if (fresh[s]) commit = 0;
+    fresh[s] = 0;
+  }
   if (commit) {
     sc.alpha[s] = sc.alpha_new[s];
     sc.alpha_old[s] = sc.alpha_new[s];
@@ -485,11 +492,20 @@ __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restric
   out[s] = keep;
 }
 
-// init: mu_ij, shift scalars, seed scalars, r_0 = r0~ = b, everything else 0
+// init: mu_ij, shift scalars, seed scalars of system s from the shifts of
+// row q of sigma_all (bicgstab.f90:345-348, 400-483)
+__device__ void init_scalars_one(int s, int q, int nsh, int l, const Scal &sc,
+                                 const double2 *__restrict__ sigma_all, int nshift_all);
+
 __global__ void k_init_scalars(int nsys, int nsh, int l, Scal sc,
                                const double2 *__restrict__ sigma_all, int nshift_all) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsys) return;
+  init_scalars_one(s, s, nsh, l, sc, sigma_all, nshift_all);
+}
+
+__device__ void init_scalars_one(int s, int q, int nsh, int l, const Scal &sc,
+                                 const double2 *__restrict__ sigma_all, int nshift_all) {
   const double2 one = mk(1, 0);
   sc.rho[s] = mk(0, 0);
   sc.rho_old[s] = one;
@@ -497,14 +513,14 @@ __global__ void k_init_scalars(int nsys, int nsh, int l, Scal sc,
   sc.alpha[s] = mk(0, 0);
   sc.omega[s] = one;
   sc.beta[s] = mk(0, 0);
-  const double2 s1 = sigma_all[(int64_t)s * nshift_all];
+  const double2 s1 = sigma_all[(int64_t)q * nshift_all];
   const int nmu = l * (l + 1) / 2;
   for (int k = 0; k < nsh; ++k) {
     int idx = s * nsh + k;
     sc.inv_phi_old[idx] = one;
     sc.inv_phi[idx] = one;
     sc.inv_theta[idx] = one;
-    double2 sg = csub(sigma_all[(int64_t)s * nshift_all + k + 1], s1);
+    double2 sg = csub(sigma_all[(int64_t)q * nshift_all + k + 1], s1);
     sc.sigma[idx] = sg;
     double2 spow[kMaxL + 1];
     spow[0] = one;
@@ -519,6 +535,61 @@ __global__ void k_init_scalars(int nsys, int nsh, int l, Scal sc,
       }
     }
   }
+}
+
+// Streaming refill: pairs[2p] = slot s, pairs[2p + 1] = queued system q.
+// Vectors of slot s start as for a new solve: r_0 = r0~ = b_q, all else 0.
+__global__ void k_refill_vectors(const int32_t *__restrict__ pairs, Vecs v,
+                                 const double2 *__restrict__ rhs, int64_t rhs_ld) {
+  const int s = pairs[2 * blockIdx.y], q = pairs[2 * blockIdx.y + 1];
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  const double2 b = rhs[(int64_t)q * rhs_ld + e], z = mk(0, 0);
+  for (int i = 0; i <= v.l; ++i) {
+    v.r(s, i)[e] = i == 0 ? b : z;
+    v.u(s, i)[e] = z;
+  }
+  v.x(s)[e] = z;
+  v.RT[(int64_t)s * v.ld + e] = b;
+  for (int k = 0; k < v.nsh; ++k) {
+    for (int i = 0; i <= v.l; ++i) v.su(s, k, i)[e] = z;
+    v.sx(s, k)[e] = z;
+  }
+}
+
+// Scalars and flags of the refilled slots: the seed shift of q, a fresh
+// first iteration, active.
+__global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, int nsh, int l,
+                                 Scal sc, const double2 *__restrict__ sigma_all, int nshift_all,
+                                 double2 *__restrict__ sigma_seed, int32_t *__restrict__ active,
+                                 int32_t *__restrict__ iters, int32_t *__restrict__ state,
+                                 int32_t *__restrict__ fresh) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int s = pairs[2 * p], q = pairs[2 * p + 1];
+  init_scalars_one(s, q, nsh, l, sc, sigma_all, nshift_all);
+  sigma_seed[s] = sigma_all[(int64_t)q * nshift_all];
+  active[s] = 1;
+  iters[s] = 0;
+  state[s] = 0;
+  fresh[s] = 1;
+}
+
+// slots whose system ran out of iterations stop (pairs as above)
+__global__ void k_deactivate(int npairs, const int32_t *__restrict__ pairs,
+                             int32_t *__restrict__ active) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npairs) active[pairs[2 * p]] = 0;
+}
+
+// x_out[q][k][ldo] <- X / SX of slot s for every pair (s, q)
+__global__ void k_gather_pairs(const int32_t *__restrict__ pairs, Vecs v, double2 *__restrict__ out,
+                               int64_t ldo, int64_t n) {
+  const int s = pairs[2 * blockIdx.y], q = pairs[2 * blockIdx.y + 1], k = blockIdx.z;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= n) return;
+  const double2 val = (k == 0) ? v.x(s)[e] : v.sx(s, k - 1)[e];
+  out[((int64_t)q * (v.nsh + 1) + k) * ldo + e] = val;
 }
 
 __global__ void k_init_vectors(int nsys, Vecs v, const double2 *__restrict__ rhs, int64_t rhs_ld) {
@@ -596,9 +667,15 @@ struct BatchBicgstab::Impl {
   std::vector<int32_t> h_active;
   int32_t *pinned_active = nullptr;  // 2 x nsys flags for the lagged convergence test
   hipEvent_t flag_ev[2] = {nullptr, nullptr};
+  // streaming solves: first-iteration flags, the fixed group of each slot,
+  // (slot, system) lists on the device (4 x 2 nsys) and pinned host staging
+  // (2 x 3 nsys flag copies, then 4 x 2 nsys lists)
+  DevBuf<int32_t> fresh, slot_group, pairs_dev;
+  int32_t *pinned_stream = nullptr;
 
   ~Impl() {
     if (pinned_active) (void)hipHostFree(pinned_active);
+    if (pinned_stream) (void)hipHostFree(pinned_stream);
     for (auto &e : flag_ev)
       if (e) (void)hipEventDestroy(e);
   }
@@ -674,6 +751,58 @@ struct BatchBicgstab::Impl {
                                                state.p, sc, nsh, shift_test, relax_thr,
                                                relax_thr > 0 ? relax.p : nullptr);
   }
+
+  // One outer iteration (bicgstab.f90:229-247) on every active system.
+  // `fresh` marks systems in their first iteration (streaming), else null.
+  void iterate(Operator &op, hipStream_t st, int iter, double threshold, int32_t *fresh) {
+    const int S = nsys;
+    const dim3 gv((unsigned)((ld + kBlock - 1) / kBlock), S);
+    const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
+    const int64_t sstride = (int64_t)(l + 1) * ld;
+    // ---- bicg_part (Algorithm 3)
+    k_bicg_begin<<<grid1(S, 64), 64, 0, st>>>(S, active.p, sc);
+    for (int j = 0; j < l; ++j) {
+      {
+        KScope ks_("dotu", st, 2);
+        k_dotu<<<S, kBlock, 0, st>>>(S, active.p, R.p + j * ld, sstride, RT.p, ld, ld, sc.dot);
+      }
+      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, (iter > 1 || j > 0) ? 1 : 0, active.p, sc,
+                                              j == 0 ? fresh : nullptr);
+      {
+        KScope ks_("bicg_v1", st, 2);
+        k_bicg_v1<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
+      }
+      op_apply(op, st, U.p + j * ld, U.p + (j + 1) * ld);       // L10
+      {
+        KScope ks_("dotu", st, 2);
+        k_dotu<<<S, kBlock, 0, st>>>(S, active.p, U.p + (j + 1) * ld, sstride, RT.p, ld, ld,
+                                     sc.dot);
+      }
+      k_bicg_s2<<<grid1(S * std::max(nsh, 1), 64), 64, 0, st>>>(S, nsh, l, j, active.p, sc);
+      {
+        KScope ks_("bicg_seed", st, 2);
+        k_bicg_seed<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
+      }
+      op_apply(op, st, R.p + j * ld, R.p + (j + 1) * ld);       // L24
+    }
+    check(st, iter, threshold, active.p, mr_active.p);
+    // ---- mr_part (Algorithm 4) on the systems still running
+    {
+      KScope ks_("mr_mgs", st, 2);
+      k_mr_mgs<1024><<<S, 1024, 0, st>>>(mr_active.p, vv, sc, ntau, nmu);
+    }
+    if (nsh > 0) {  // the shifted systems: BiCG half of this iteration and, if running, MR half
+      KScope ks_("shift_cycle", st, 2);
+      const dim3 gs(gv.x, gv.y, (unsigned)((nsh + kShiftChunk - 1) / kShiftChunk));
+      launch_l<ShiftCycle>(l, gs, dim3(kBlock), st, (const int32_t *)active.p,
+                           (const int32_t *)mr_active.p, vv, sc);
+    }
+    {
+      KScope ks_("mr_seed", st, 2);
+      launch_l<MrSeed>(l, gv, dim3(kBlock), st, (const int32_t *)mr_active.p, vv, sc);
+    }
+    check(st, iter, threshold, mr_active.p, active.p);
+  }
 };
 
 BatchBicgstab::BatchBicgstab(int nsys, int nshift, int lmax, int64_t ld)
@@ -734,56 +863,11 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   const dim3 gv((unsigned)((m.ld + kBlock - 1) / kBlock), S);
   k_init_vectors<<<gv, kBlock, 0, st>>>(S, m.vv, d_rhs, rhs_ld);
   k_init_scalars<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, l, m.sc, d_sigma, m.nshift);
-  const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
-  const int64_t sstride = (int64_t)(l + 1) * m.ld;
   const bool host_op = op.host_side();
   int iter = 0;
   int last_iter = 0;
   for (iter = 1; iter <= max_iter; ++iter) {
-    // ---- bicg_part (Algorithm 3)
-    k_bicg_begin<<<grid1(S, 64), 64, 0, st>>>(S, m.active.p, m.sc);
-    for (int j = 0; j < l; ++j) {
-      {
-        KScope ks_("dotu", st, 2);
-        k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.R.p + j * m.ld, sstride, m.RT.p, m.ld, m.ld,
-                                     m.sc.dot);
-      }
-      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, (iter > 1 || j > 0) ? 1 : 0, m.active.p, m.sc);
-      {
-        KScope ks_("bicg_v1", st, 2);
-        k_bicg_v1<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
-      }
-      m.op_apply(op, st, m.U.p + j * m.ld, m.U.p + (j + 1) * m.ld);       // L10
-      {
-        KScope ks_("dotu", st, 2);
-        k_dotu<<<S, kBlock, 0, st>>>(S, m.active.p, m.U.p + (j + 1) * m.ld, sstride, m.RT.p, m.ld,
-                                     m.ld, m.sc.dot);
-      }
-      k_bicg_s2<<<grid1(S * std::max(m.nsh, 1), 64), 64, 0, st>>>(S, m.nsh, l, j, m.active.p,
-                                                                   m.sc);
-      {
-        KScope ks_("bicg_seed", st, 2);
-        k_bicg_seed<<<gv, kBlock, 0, st>>>(j, m.active.p, m.vv, m.sc);
-      }
-      m.op_apply(op, st, m.R.p + j * m.ld, m.R.p + (j + 1) * m.ld);       // L24
-    }
-    m.check(st, iter, threshold, m.active.p, m.mr_active.p);
-    // ---- mr_part (Algorithm 4) on the systems still running
-    {
-      KScope ks_("mr_mgs", st, 2);
-      k_mr_mgs<1024><<<S, 1024, 0, st>>>(m.mr_active.p, m.vv, m.sc, ntau, nmu);
-    }
-    if (m.nsh > 0) {  // the shifted systems: BiCG half of this iteration and, if running, MR half
-      KScope ks_("shift_cycle", st, 2);
-      const dim3 gs(gv.x, gv.y, (unsigned)((m.nsh + kShiftChunk - 1) / kShiftChunk));
-      launch_l<ShiftCycle>(l, gs, dim3(kBlock), st, (const int32_t *)m.active.p,
-                           (const int32_t *)m.mr_active.p, m.vv, m.sc);
-    }
-    {
-      KScope ks_("mr_seed", st, 2);
-      launch_l<MrSeed>(l, gv, dim3(kBlock), st, (const int32_t *)m.mr_active.p, m.vv, m.sc);
-    }
-    m.check(st, iter, threshold, m.mr_active.p, m.active.p);
+    m.iterate(op, st, iter, threshold, nullptr);
     last_iter = iter;
     if (host_op) {  // host operators read the active flags on every call
       m.sync_active(st);
@@ -838,6 +922,183 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
     if (nan_h[s]) ierr = 2;  // bicgstab.f90:264-267
     if (h_ierr) h_ierr[s] = ierr;
     if (h_iters) h_iters[s] = its;
+  }
+}
+
+void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const double2 *d_rhs,
+                                 int64_t rhs_ld, const double2 *d_sigma, const double2 *h_sigma,
+                                 const int32_t *h_group, double threshold, int max_iter,
+                                 double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
+                                 int shift_test) {
+  Impl &m = *impl_;
+  const int S = m.nsys;
+  SGW_REQUIRE(!op.host_side(), SGW_E_ARG, "bicgstab stream: device-resident operators only");
+  SGW_REQUIRE(nq >= 0 && h_group && rhs_ld >= m.ld, SGW_E_ARG,
+              "bicgstab stream: need nq >= 0, host group indices and rhs_ld >= ld");
+  m.shift_test = shift_test;
+  m.relax_thr = 0.0;
+  // slots per group, in proportion to the queued systems of each group
+  const int ng = op.ngroups();
+  std::vector<std::vector<int>> queue(ng);
+  for (int q = 0; q < nq; ++q) {
+    SGW_REQUIRE(h_group[q] >= 0 && h_group[q] < ng, SGW_E_ARG,
+                "bicgstab stream: group index out of range");
+    queue[h_group[q]].push_back(q);
+  }
+  int present = 0;
+  for (auto &qu : queue) present += qu.empty() ? 0 : 1;
+  SGW_REQUIRE(present <= S, SGW_E_SHAPE, "bicgstab stream: more groups than slots");
+  std::vector<int> share(ng, 0);
+  int used = 0;
+  for (int g = 0; g < ng; ++g)
+    if (!queue[g].empty()) {
+      share[g] = std::max(1, (int)((int64_t)S * (int64_t)queue[g].size() / std::max(nq, 1)));
+      share[g] = std::min<int>(share[g], (int)queue[g].size());
+      used += share[g];
+    }
+  while (used > S) {  // trim the largest shares
+    int g = (int)(std::max_element(share.begin(), share.end()) - share.begin());
+    --share[g];
+    --used;
+  }
+  std::vector<int32_t> slot_group(S, 0);
+  std::vector<int> slot_q(S, -1), slot_start(S, 0), head(ng, 0);
+  {
+    int s = 0;
+    for (int g = 0; g < ng; ++g)
+      for (int k = 0; k < share[g]; ++k) slot_group[s++] = g;
+    for (int t = s; t < S; ++t) slot_group[t] = slot_group[0];  // idle slots
+  }
+  if (m.fresh.n < (size_t)S) m.fresh.alloc(S);
+  if (m.slot_group.n < (size_t)S) m.slot_group.alloc(S);
+  if (m.pairs_dev.n < (size_t)8 * S) m.pairs_dev.alloc((size_t)8 * S);
+  if (!m.pinned_stream) {
+    SGW_HIP(hipHostMalloc((void **)&m.pinned_stream, sizeof(int32_t) * 14 * S,
+                          hipHostMallocDefault));
+  }
+  int32_t *h_flags = m.pinned_stream;          // 2 x [active | state | iters] (3 S each)
+  int32_t *h_pairs = m.pinned_stream + 6 * S;  // 4 x 2 S lists
+  // List buffer b (0..3) is written by the host and copied to the device on
+  // the stream.  At host iteration it the gather list uses b = it & 1 and the
+  // refill list b = 2 + (it & 1); their previous copies were enqueued at
+  // it - 2, before iteration it - 1, whose flag event the host has waited
+  // for before writing them again.
+  auto stage = [&](const std::vector<std::pair<int, int>> &pr, int b) -> int32_t * {
+    int32_t *hp = h_pairs + (size_t)b * 2 * S;
+    for (size_t i = 0; i < pr.size(); ++i) {
+      hp[2 * i] = pr[i].first;
+      hp[2 * i + 1] = pr[i].second;
+    }
+    int32_t *dp = m.pairs_dev.p + (size_t)b * 2 * S;
+    SGW_HIP(hipMemcpyAsync(dp, hp, sizeof(int32_t) * 2 * pr.size(), hipMemcpyHostToDevice, st));
+    return dp;
+  };
+  m.slot_group.upload(slot_group.data(), S, st);
+  op.prepare(slot_group.data(), S);  // fixed group per slot: one plan for the whole stream
+  m.group_dev = m.slot_group.p;
+  m.h_sigma_seed.assign(S, make_double2(0, 0));
+  std::vector<int32_t> zero(S, 0);
+  m.h_active.assign(S, 1);  // host mirror unused by device operators
+  m.active.upload(zero.data(), S, st);
+  m.mr_active.upload(zero.data(), S, st);
+  m.fresh.upload(zero.data(), S, st);
+  SGW_HIP(hipMemsetAsync(m.iters.p, 0, sizeof(int32_t) * S, st));
+  SGW_HIP(hipMemsetAsync(m.state.p, 0, sizeof(int32_t) * S, st));
+  for (int q = 0; q < nq; ++q) {
+    if (h_ierr) h_ierr[q] = 1;
+    if (h_iters) h_iters[q] = max_iter + 1;
+  }
+  const int64_t nout = std::min<int64_t>(x_ld, m.ld);
+  const unsigned tiles = (unsigned)((m.ld + kBlock - 1) / kBlock);
+  const unsigned otiles = (unsigned)((nout + kBlock - 1) / kBlock);
+  // enqueue refills of the listed (slot, system) pairs through list buffer b
+  auto refill = [&](const std::vector<std::pair<int, int>> &pr, int b) {
+    if (pr.empty()) return;
+    int32_t *dp = stage(pr, b);
+    k_refill_vectors<<<dim3(tiles, (unsigned)pr.size()), kBlock, 0, st>>>(dp, m.vv, d_rhs,
+                                                                           rhs_ld);
+    k_refill_scalars<<<grid1((int)pr.size(), 64), 64, 0, st>>>(
+        (int)pr.size(), dp, m.nsh, m.l, m.sc, d_sigma, m.nshift, m.sigma_seed.p, m.active.p,
+        m.iters.p, m.state.p, m.fresh.p);
+  };
+  // first fill
+  std::vector<std::pair<int, int>> pr;
+  for (int s = 0; s < used; ++s) {
+    const int g = slot_group[s];
+    if (head[g] < (int)queue[g].size()) {
+      slot_q[s] = queue[g][head[g]++];
+      slot_start[s] = 1;
+      pr.push_back({s, slot_q[s]});
+    }
+  }
+  refill(pr, 3);  // list buffer 3 is next written at host iteration 3
+  int running = (int)pr.size();
+  for (int it = 1; running > 0; ++it) {
+    m.iterate(op, st, it, threshold, m.fresh.p);
+    const int slot = it & 1;
+    int32_t *hf = h_flags + (size_t)slot * 3 * S;
+    SGW_HIP(hipMemcpyAsync(hf, m.active.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipMemcpyAsync(hf + S, m.state.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipMemcpyAsync(hf + 2 * S, m.iters.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost,
+                           st));
+    SGW_HIP(hipEventRecord(m.flag_ev[slot], st));
+    // look at the previous iteration's flags while this one runs (at it = 1
+    // there is none; a final pass below drains the last iteration)
+    const int p = it - 1;
+    if (p < 1) continue;
+    SGW_HIP(hipEventSynchronize(m.flag_ev[slot ^ 1]));
+    const int32_t *fa = h_flags + (size_t)(slot ^ 1) * 3 * S;
+    std::vector<std::pair<int, int>> done, forced, fill;
+    for (int s = 0; s < S; ++s) {
+      if (slot_q[s] < 0 || slot_start[s] > p) continue;
+      const int q = slot_q[s];
+      if (!fa[s]) {  // converged or NaN at iteration <= p
+        if (h_iters) h_iters[q] = fa[2 * S + s] - slot_start[s] + 1;
+        if (h_ierr) h_ierr[q] = fa[S + s] == 1 ? 0 : 1;
+        done.push_back({s, q});
+      } else if (p - slot_start[s] + 1 >= max_iter) {  // ierr 1 (bicgstab.f90:249-253)
+        forced.push_back({s, q});
+        done.push_back({s, q});
+      }
+    }
+    if (done.empty()) continue;
+    if (!forced.empty()) {  // rare: stop them, then the buffer is free again
+      int32_t *dp = stage(forced, slot);
+      k_deactivate<<<grid1((int)forced.size(), 64), 64, 0, st>>>((int)forced.size(), dp,
+                                                                  m.active.p);
+      SGW_HIP(hipStreamSynchronize(st));
+    }
+    {  // solutions of the finished systems
+      int32_t *dp = stage(done, slot);
+      k_gather_pairs<<<dim3(otiles, (unsigned)done.size(), m.nshift), kBlock, 0, st>>>(
+          dp, m.vv, d_x, x_ld, nout);
+    }
+    for (auto &d : done) {
+      const int s = d.first, g = slot_group[s];
+      if (head[g] < (int)queue[g].size()) {
+        slot_q[s] = queue[g][head[g]++];
+        slot_start[s] = it + 1;
+        fill.push_back({s, slot_q[s]});
+      } else {
+        slot_q[s] = -1;
+        --running;
+      }
+    }
+    refill(fill, 2 + slot);
+  }
+  // NaN scan of every solution (bicgstab.f90:264-267)
+  if (nq > 0) {
+    DevBuf<int32_t> nanflag(nq);
+    nanflag.zero(st);
+    k_nan_scan<<<nq, kBlock, 0, st>>>(nq, m.nshift, d_x, x_ld, nout, nanflag.p);
+    std::vector<int32_t> nan_h(nq);
+    SGW_HIP(hipMemcpyAsync(nan_h.data(), nanflag.p, sizeof(int32_t) * nq, hipMemcpyDeviceToHost,
+                           st));
+    SGW_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < nq; ++q)
+      if (nan_h[q] && h_ierr) h_ierr[q] = 2;
+  } else {
+    SGW_HIP(hipStreamSynchronize(st));
   }
 }
 

@@ -354,6 +354,27 @@ int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
   });
 }
 
+int sgw_batch_solver_run_stream(sgw_batch_solver *solver, int32_t nq, const int32_t *h_group,
+                                const sgw_complex *d_rhs, const sgw_complex *d_sigma,
+                                const sgw_complex *h_sigma, sgw_complex *d_x, int32_t *h_ierr,
+                                int32_t *h_iters, void *stream) {
+  return guarded([&] {
+    SGW_REQUIRE(solver && h_group && d_rhs && d_sigma && h_sigma && d_x, SGW_E_ARG,
+                "batch_solver_run_stream: null argument");
+    SGW_REQUIRE(nq >= 0, SGW_E_ARG, "batch_solver_run_stream: negative queue length");
+    const sgw_batch_config &c = solver->cfg;
+    std::vector<int32_t> ierr(std::max(nq, 1)), it(std::max(nq, 1));
+    solver->engine->solve_stream(*solver->op->op, (hipStream_t)stream, nq, D2(d_rhs), c.ld,
+                                 D2(d_sigma), D2(h_sigma), h_group, c.threshold, c.max_iter,
+                                 D2(d_x), c.ld, ierr.data(), it.data(), c.shift_test);
+    int worst = 0;
+    for (int q = 0; q < nq; ++q) worst = std::max(worst, ierr[q]);
+    if (h_ierr && nq > 0) std::memcpy(h_ierr, ierr.data(), sizeof(int32_t) * nq);
+    if (h_iters && nq > 0) std::memcpy(h_iters, it.data(), sizeof(int32_t) * nq);
+    return worst;
+  });
+}
+
 int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
                                   const int32_t *group, const sgw_complex *rhs,
                                   const sgw_complex *sigma, sgw_complex *x, int32_t *ierr,

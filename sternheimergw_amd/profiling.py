@@ -6,7 +6,7 @@
         per-kernel average counter value (e.g. FETCH_SIZE or WRITE_SIZE, KiB
         per dispatch); with copy-bytes, the largest copy kernel's reading is
         used to scale KiB into bytes (calibration against a copy of known size)
-    python -m sternheimergw_amd.profiling happly <trace.db> <fetch.db> <write.db> <copy-bytes>
+    python -m sternheimergw_amd.profiling happly <trace.db> <fetch.db> <write.db> <copy-bytes> [skip]
         the plane-wave H-apply as one unit (all k_pw_* kernels of one
         application): average duration per application from the kernel trace
         and calibrated HBM bytes read and written per application, as JSON
@@ -75,33 +75,42 @@ def pmc(db: str, copy_bytes: int = 0) -> str:
     return "\n".join(out)
 
 
-def _pmc_per_kernel(db: str, copy_bytes: int):
+def _pmc_per_kernel(db: str, copy_bytes: int, skip_last: int = 0):
     con = sqlite3.connect(db)
-    rows = con.execute("select kernel_name, counter_name, value from counters_collection").fetchall()
+    rows = con.execute("select kernel_name, counter_name, value from counters_collection "
+                       "order by dispatch_id").fetchall()
     agg = collections.defaultdict(list)
     for name, _c, val in rows:
         agg[name].append(val)
     copies = [max(v) for n, v in agg.items() if "copy" in n.lower() or "elementwise" in n.lower()]
+    if skip_last:
+        agg = {n: (v[:-skip_last] if "k_pw_" in n and len(v) > skip_last else v)
+               for n, v in agg.items()}
     scale = copy_bytes / (max(copies) * 1024.0) if copies and max(copies) else None
     return agg, scale
 
 
-def happly(trace_db: str, fetch_db: str, write_db: str, copy_bytes: int) -> dict:
+def happly(trace_db: str, fetch_db: str, write_db: str, copy_bytes: int,
+           skip_last: int = 0) -> dict:
     """The H-apply's kernels launch once each per application, so the
-    application's average is the sum of the per-kernel averages."""
+    application's average is the sum of the per-kernel averages.  skip_last
+    leaves out the last applications of each database (the residual checks
+    that follow the timed solve)."""
     con = sqlite3.connect(trace_db)
-    rows = con.execute("select name, duration from kernels").fetchall()
+    rows = con.execute("select name, duration from kernels order by start").fetchall()
     dur = collections.defaultdict(list)
     for name, d in rows:
         if "k_pw_" in name:
             dur[_short(name)].append(d)
+    if skip_last:
+        dur = {k: v[:-skip_last] for k, v in dur.items() if len(v) > skip_last}
     launches = {k: len(v) for k, v in dur.items()}
     avg_ns = {k: sum(v) / len(v) for k, v in dur.items()}
     out = {"kernels": sorted(dur), "launches": launches,
            "avg_us": {k: round(v / 1e3, 3) for k, v in avg_ns.items()},
            "happly_avg_ms": round(sum(avg_ns.values()) / 1e6, 5)}
     for tag, db in (("fetch", fetch_db), ("write", write_db)):
-        agg, scale = _pmc_per_kernel(db, copy_bytes)
+        agg, scale = _pmc_per_kernel(db, copy_bytes, skip_last)
         per = {_short(n): sum(v) / len(v) * 1024.0 * (scale or 1.0)
                for n, v in agg.items() if "k_pw_" in n}
         out[f"{tag}_factor"] = round(scale, 4) if scale else None
@@ -117,7 +126,8 @@ def main(argv):
         return 2
     if argv[1] == "happly":
         import json
-        print(json.dumps(happly(argv[2], argv[3], argv[4], int(argv[5])), indent=1))
+        skip = int(argv[6]) if len(argv) > 6 else 0
+        print(json.dumps(happly(argv[2], argv[3], argv[4], int(argv[5]), skip), indent=1))
     elif argv[1] == "trace":
         print(trace(argv[2]))
     else:
