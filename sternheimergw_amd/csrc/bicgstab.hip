@@ -493,9 +493,12 @@ __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restric
 }
 
 // init: mu_ij, shift scalars, seed scalars of system s from the shifts of
-// row q of sigma_all (bicgstab.f90:345-348, 400-483)
+// row q of sigma_all (bicgstab.f90:345-348, 400-483); init_shift_one does the
+// part of shifted system k alone
 __device__ void init_scalars_one(int s, int q, int nsh, int l, const Scal &sc,
                                  const double2 *__restrict__ sigma_all, int nshift_all);
+__device__ void init_shift_one(int s, int q, int k, int nsh, int l, const Scal &sc,
+                               const double2 *__restrict__ sigma_all, int nshift_all);
 
 __global__ void k_init_scalars(int nsys, int nsh, int l, Scal sc,
                                const double2 *__restrict__ sigma_all, int nshift_all) {
@@ -513,26 +516,30 @@ __device__ void init_scalars_one(int s, int q, int nsh, int l, const Scal &sc,
   sc.alpha[s] = mk(0, 0);
   sc.omega[s] = one;
   sc.beta[s] = mk(0, 0);
+  for (int k = 0; k < nsh; ++k) init_shift_one(s, q, k, nsh, l, sc, sigma_all, nshift_all);
+}
+
+__device__ void init_shift_one(int s, int q, int k, int nsh, int l, const Scal &sc,
+                               const double2 *__restrict__ sigma_all, int nshift_all) {
+  const double2 one = mk(1, 0);
   const double2 s1 = sigma_all[(int64_t)q * nshift_all];
   const int nmu = l * (l + 1) / 2;
-  for (int k = 0; k < nsh; ++k) {
-    int idx = s * nsh + k;
-    sc.inv_phi_old[idx] = one;
-    sc.inv_phi[idx] = one;
-    sc.inv_theta[idx] = one;
-    double2 sg = csub(sigma_all[(int64_t)q * nshift_all + k + 1], s1);
-    sc.sigma[idx] = sg;
-    double2 spow[kMaxL + 1];
-    spow[0] = one;
-    for (int i = 1; i <= l - 1; ++i) spow[i] = cmul(sg, spow[i - 1]);
-    double bin[(kMaxL + 1) * (kMaxL + 2) / 2];
-    for (int jj = 0; jj <= l - 1; ++jj) {
-      int offset = jj * (jj + 1) / 2 + 1;
-      for (int ii = 0; ii <= jj; ++ii) {
-        int ij = offset + ii;
-        bin[ij - 1] = (ii == 0) ? 1.0 : (bin[ij - 2] * (jj - ii + 1)) / ii;
-        sc.mu[(int64_t)idx * nmu + ij - 1] = cmul(mk(bin[ij - 1], 0.0), spow[jj - ii]);
-      }
+  const int idx = s * nsh + k;
+  sc.inv_phi_old[idx] = one;
+  sc.inv_phi[idx] = one;
+  sc.inv_theta[idx] = one;
+  double2 sg = csub(sigma_all[(int64_t)q * nshift_all + k + 1], s1);
+  sc.sigma[idx] = sg;
+  double2 spow[kMaxL + 1];
+  spow[0] = one;
+  for (int i = 1; i <= l - 1; ++i) spow[i] = cmul(sg, spow[i - 1]);
+  double bin[(kMaxL + 1) * (kMaxL + 2) / 2];
+  for (int jj = 0; jj <= l - 1; ++jj) {
+    int offset = jj * (jj + 1) / 2 + 1;
+    for (int ii = 0; ii <= jj; ++ii) {
+      int ij = offset + ii;
+      bin[ij - 1] = (ii == 0) ? 1.0 : (bin[ij - 2] * (jj - ii + 1)) / ii;
+      sc.mu[(int64_t)idx * nmu + ij - 1] = cmul(mk(bin[ij - 1], 0.0), spow[jj - ii]);
     }
   }
 }
@@ -564,10 +571,15 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
                                  double2 *__restrict__ sigma_seed, int32_t *__restrict__ active,
                                  int32_t *__restrict__ iters, int32_t *__restrict__ state,
                                  int32_t *__restrict__ fresh) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
+  // one thread per (pair, shifted system); the k = 0 thread also does the seed
+  const int nshw = nsh > 0 ? nsh : 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs * nshw) return;
+  const int p = t / nshw, k = t % nshw;
   const int s = pairs[2 * p], q = pairs[2 * p + 1];
-  init_scalars_one(s, q, nsh, l, sc, sigma_all, nshift_all);
+  if (nsh > 0) init_shift_one(s, q, k, nsh, l, sc, sigma_all, nshift_all);
+  if (k != 0) return;
+  init_scalars_one(s, q, 0, l, sc, sigma_all, nshift_all);  // seed scalars only
   sigma_seed[s] = sigma_all[(int64_t)q * nshift_all];
   active[s] = 1;
   iters[s] = 0;
@@ -1017,7 +1029,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
     int32_t *dp = stage(pr, b);
     k_refill_vectors<<<dim3(tiles, (unsigned)pr.size()), kBlock, 0, st>>>(dp, m.vv, d_rhs,
                                                                            rhs_ld);
-    k_refill_scalars<<<grid1((int)pr.size(), 64), 64, 0, st>>>(
+    k_refill_scalars<<<grid1((int)pr.size() * std::max(m.nsh, 1), 64), 64, 0, st>>>(
         (int)pr.size(), dp, m.nsh, m.l, m.sc, d_sigma, m.nshift, m.sigma_seed.p, m.active.p,
         m.iters.p, m.state.p, m.fresh.p);
   };

@@ -180,18 +180,29 @@ class LinterSetup:
         self.xq = sq.kvec - w.sphere_k.kvec
 
 
+SLOTS = 256  # systems solved side by side; larger batches stream through them
+
+
 def _solve(setup, d_rhs, sigma, threshold, lmax, max_iter, shift_test, stats=None):
     """Batched multishift solve of rhs (nsys, ld) at sigma (nsys, nshift);
-    adds the vectors the operator was applied to into stats."""
+    adds the vectors the operator was applied to into stats.  More than
+    SLOTS systems stream through SLOTS slots (sgw_batch_solver_run_stream)."""
     import torch
     nsys, nshift = sigma.shape
     dev = d_rhs.device
-    bs = BatchSolver(setup.op, nsys, nshift, setup.ld, lmax=lmax, max_iter=max_iter,
-                     threshold=threshold, shift_test=shift_test)
-    d_group = torch.full((nsys,), setup.group, dtype=torch.int32, device=dev)
-    d_sigma = torch.from_numpy(np.ascontiguousarray(sigma)).to(dev)
+    sigma = np.ascontiguousarray(sigma)
+    d_sigma = torch.from_numpy(sigma).to(dev)
     d_x = torch.zeros((nsys, nshift, setup.ld), dtype=torch.complex128, device=dev)
-    ierr, iters = bs.solve(d_group, d_rhs, d_sigma, sigma, d_x)
+    if nsys > SLOTS:
+        bs = BatchSolver(setup.op, SLOTS, nshift, setup.ld, lmax=lmax, max_iter=max_iter,
+                         threshold=threshold, shift_test=shift_test)
+        ierr, iters = bs.solve_stream(np.full(nsys, setup.group, dtype=np.int32), d_rhs, d_sigma,
+                                      sigma, d_x)
+    else:
+        bs = BatchSolver(setup.op, nsys, nshift, setup.ld, lmax=lmax, max_iter=max_iter,
+                         threshold=threshold, shift_test=shift_test)
+        d_group = torch.full((nsys,), setup.group, dtype=torch.int32, device=dev)
+        ierr, iters = bs.solve(d_group, d_rhs, d_sigma, sigma, d_x)
     if np.any(ierr != 0):
         raise SolverError("solver did not converge")  # errore, solve_linter.f90:370
     if stats is not None:

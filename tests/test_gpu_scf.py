@@ -236,3 +236,24 @@ def test_coulomb_columns_batched_equal_one_at_a_time(linter):
         col, _it = scf.coulomb_column(setup, pm, freq, 1, g_mill, bg=b, gcut2=gcut2,
                                       threshold=1e-11)
         assert np.abs(eps[p] - col).max() <= 1e-8 * np.abs(col).max(), p
+
+
+def test_coulomb_columns_streamed_beyond_the_slots(linter):
+    """65 perturbations x 4 bands = 260 systems: more than the 256 slots, so
+    the solve streams; the columns equal the ones computed one at a time."""
+    w, _op, setup, b, omega = linter
+    freq = np.array([0.0, 1.0j])
+    gcut2 = 2 * 4 * 6.0
+    r = np.arange(-2, 3)
+    allm = np.array(np.meshgrid(r, r, r, indexing="ij")).reshape(3, -1).T
+    allm = allm[np.any(allm != 0, axis=1)]
+    pms = allm[np.argsort((allm ** 2).sum(1), kind="stable")][:65]
+    assert len(pms) * w.nbnd > scf.SLOTS
+    g_mill = np.vstack([[0, 0, 0], pms[:4]])
+    print("coulomb columns: 65 perturbations streamed through 256 slots", flush=True)
+    eps = scf.coulomb_columns_direct(setup, pms, freq, g_mill, bg=b, gcut2=gcut2,
+                                     threshold=1e-11)
+    for p in (0, 64):
+        col, _it = scf.coulomb_column(setup, pms[p], freq, 1, g_mill, bg=b, gcut2=gcut2,
+                                      threshold=1e-11)
+        assert np.abs(eps[p] - col).max() <= 1e-8 * np.abs(col).max(), p
