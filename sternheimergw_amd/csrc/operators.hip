@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <type_traits>
 
@@ -201,6 +203,12 @@ __device__ inline void stage_plane(const C *__restrict__ src, C *dst, int n) {
 // e^{+2 pi i m3 z / N}.  One lane per column (m1, m2); the column's plane
 // waves are consecutive in sorted order with m3 = lo .. lo + cnt - 1 (the
 // sphere is convex).
+template <int N, typename C>
+__device__ inline void zinv_body(const PwDev &d, const double2 *__restrict__ in, int64_t sin,
+                                 const int32_t *__restrict__ group,
+                                 const int32_t *__restrict__ active, C *__restrict__ T,
+                                 const int32_t *__restrict__ relax, int want, int v, int c);
+
 template <int N, typename C = double2>
 __global__ __launch_bounds__(64) void k_pw_zinv(PwDev d, const double2 *__restrict__ in,
                                                 int64_t sin, const int32_t *__restrict__ group,
@@ -208,7 +216,15 @@ __global__ __launch_bounds__(64) void k_pw_zinv(PwDev d, const double2 *__restri
                                                 C *__restrict__ T,
                                                 const int32_t *__restrict__ relax = nullptr,
                                                 int want = 0) {
-  const int v = blockIdx.y, c = blockIdx.x * 64 + threadIdx.x;
+  zinv_body<N, C>(d, in, sin, group, active, T, relax, want, blockIdx.y,
+                  blockIdx.x * 64 + threadIdx.x);
+}
+
+template <int N, typename C>
+__device__ inline void zinv_body(const PwDev &d, const double2 *__restrict__ in, int64_t sin,
+                                 const int32_t *__restrict__ group,
+                                 const int32_t *__restrict__ active, C *__restrict__ T,
+                                 const int32_t *__restrict__ relax, int want, int v, int c) {
   if (active && !active[v]) return;
   if (precision_skip(relax, want, v)) return;
   if (c >= d.nslot) return;
@@ -279,6 +295,13 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
 //      columns' values S[v][c][z].
 // LDS holds the column values T[nc1][nc2] and one row buffer P[nc1][N] that
 // the X stage overwrites column by column (each lane owns its column y).
+template <int N, typename C>
+__device__ inline void plane_body(const PwDev &d, const C *__restrict__ Tin,
+                                  const int32_t *__restrict__ group,
+                                  const int32_t *__restrict__ active, C *__restrict__ S,
+                                  const int32_t *__restrict__ relax, int want, int z, int v,
+                                  C *smem);
+
 template <int N, typename C = double2>
 __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const C *__restrict__ Tin,
                                                      const int32_t *__restrict__ group,
@@ -286,10 +309,18 @@ __global__ __launch_bounds__(64) void k_pw_plane_fft(PwDev d, const C *__restric
                                                      C *__restrict__ S,
                                                      const int32_t *__restrict__ relax = nullptr,
                                                      int want = 0) {
-  using R = typename CplxT<C>::S;
   extern __shared__ double2 smem_d[];
-  C *smem = reinterpret_cast<C *>(smem_d);
-  const int z = blockIdx.x, v = blockIdx.y;
+  plane_body<N, C>(d, Tin, group, active, S, relax, want, blockIdx.x, blockIdx.y,
+                   reinterpret_cast<C *>(smem_d));
+}
+
+template <int N, typename C>
+__device__ inline void plane_body(const PwDev &d, const C *__restrict__ Tin,
+                                  const int32_t *__restrict__ group,
+                                  const int32_t *__restrict__ active, C *__restrict__ S,
+                                  const int32_t *__restrict__ relax, int want, int z, int v,
+                                  C *smem) {
+  using R = typename CplxT<C>::S;
   if (active && !active[v]) return;
   if (precision_skip(relax, want, v)) return;
   const int g = group ? group[v] : 0;
@@ -478,6 +509,12 @@ __device__ inline bool tile_active(const int32_t *__restrict__ active,
 
 // becp[g][vt][p][vl] = D_p sum over the chunk partials (once per tile); one
 // lane per output (p, vl), 64 outputs per block
+__device__ inline void becp_reduce_body(const PwDev &d, const int32_t *__restrict__ active,
+                                        const int32_t *__restrict__ gvec,
+                                        const int32_t *__restrict__ gcnt, int maxper, int tiles,
+                                        int nchunk, const double2 *__restrict__ part,
+                                        double2 *__restrict__ becp, int bx, int g, int vt);
+
 __global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
                                                        const int32_t *__restrict__ active,
                                                        const int32_t *__restrict__ gvec,
@@ -485,11 +522,19 @@ __global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
                                                        int maxper, int tiles, int nchunk,
                                                        const double2 *__restrict__ part,
                                                        double2 *__restrict__ becp) {
-  const int g = blockIdx.y, vt = blockIdx.z;
+  becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, blockIdx.x,
+                   blockIdx.y, blockIdx.z);
+}
+
+__device__ inline void becp_reduce_body(const PwDev &d, const int32_t *__restrict__ active,
+                                        const int32_t *__restrict__ gvec,
+                                        const int32_t *__restrict__ gcnt, int maxper, int tiles,
+                                        int nchunk, const double2 *__restrict__ part,
+                                        double2 *__restrict__ becp, int bx, int g, int vt) {
   const int cnt = gcnt[g], v0 = vt * kVT;
   if (v0 >= cnt) return;
   const int nv = min(kVT, cnt - v0), nproj = d.nproj;
-  const int o = blockIdx.x * 64 + threadIdx.x;
+  const int o = bx * 64 + threadIdx.x;
   if (o >= nproj * kVT) return;
   if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   const int nck = (d.npw[g] + kChunk - 1) / kChunk;
@@ -515,13 +560,27 @@ __global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
 }
 
 // part[((g*tiles + vt)*nchunk + chunk)][p][vl] = sum_{i in chunk} conj(proj_p(i)) x_v(i)
+__device__ inline void becp_body(const PwDev &d, const double2 *__restrict__ in, int64_t sin,
+                                 const int32_t *__restrict__ active,
+                                 const int32_t *__restrict__ gvec,
+                                 const int32_t *__restrict__ gcnt, int maxper, int tiles,
+                                 int nchunk, double2 *__restrict__ part, int chunk, int g, int vt);
+
 __global__ __launch_bounds__(256) void k_pw_becp(PwDev d, const double2 *__restrict__ in,
                                                  int64_t sin, const int32_t *__restrict__ active,
                                                  const int32_t *__restrict__ gvec,
                                                  const int32_t *__restrict__ gcnt, int maxper,
                                                  int tiles, int nchunk,
                                                  double2 *__restrict__ part) {
-  const int chunk = blockIdx.x, g = blockIdx.y, vt = blockIdx.z;
+  becp_body(d, in, sin, active, gvec, gcnt, maxper, tiles, nchunk, part, blockIdx.x, blockIdx.y,
+            blockIdx.z);
+}
+
+__device__ inline void becp_body(const PwDev &d, const double2 *__restrict__ in, int64_t sin,
+                                 const int32_t *__restrict__ active,
+                                 const int32_t *__restrict__ gvec,
+                                 const int32_t *__restrict__ gcnt, int maxper, int tiles,
+                                 int nchunk, double2 *__restrict__ part, int chunk, int g, int vt) {
   const int cnt = gcnt[g], v0 = vt * kVT;
   const int npw = d.npw[g], i0 = chunk * kChunk;
   if (v0 >= cnt || i0 >= npw) return;
@@ -635,6 +694,50 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   }
 }
 
+// Two independent stages of one FP64 application in one launch each, which
+// saves a launch boundary (and the tail of the smaller stage) per pair:
+//   k_pw_becp_zinv:    blocks [0, nbecp) are projector partial sums (256
+//                      lanes); the rest hold four 64-lane z inverse groups;
+//   k_pw_reduce_plane: blocks [0, nred) reduce the partials (64 lanes); the
+//                      rest are the plane transforms (z fastest).
+// The block index orders match the separate kernels' grids, and every branch
+// on it is uniform within a block, so the barriers of either part are safe.
+template <int N>
+__global__ __launch_bounds__(256) void k_pw_becp_zinv(
+    PwDev d, const double2 *__restrict__ in, int64_t sin, const int32_t *__restrict__ group,
+    const int32_t *__restrict__ active, const int32_t *__restrict__ gvec,
+    const int32_t *__restrict__ gcnt, int maxper, int tiles, int nchunk, int ngroup,
+    double2 *__restrict__ part, double2 *__restrict__ T, int nvec) {
+  const int nbecp = nchunk * ngroup * tiles, b = blockIdx.x;
+  if (b < nbecp) {
+    becp_body(d, in, sin, active, gvec, gcnt, maxper, tiles, nchunk, part, b % nchunk,
+              (b / nchunk) % ngroup, b / (nchunk * ngroup));
+    return;
+  }
+  const int gx = (d.nslot + 63) / 64;
+  const int zb = (b - nbecp) * 4 + (int)(threadIdx.x >> 6);
+  if (zb >= gx * nvec) return;
+  zinv_body<N, double2>(d, in, sin, group, active, T, nullptr, 0, zb / gx,
+                        (zb % gx) * 64 + (int)(threadIdx.x & 63));
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_reduce_plane(
+    PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
+    const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
+    int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
+    double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S) {
+  extern __shared__ double2 smem_d[];
+  const int nred = nred_x * ngroup * tiles, b = blockIdx.x;
+  if (b < nred) {
+    becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, b % nred_x,
+                     (b / nred_x) % ngroup, b / (nred_x * ngroup));
+    return;
+  }
+  const int pb = b - nred;
+  plane_body<N, double2>(d, T, group, active, S, nullptr, 0, pb % N, pb / N, smem_d);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -695,6 +798,7 @@ static inline int modn(int a, int n) { return ((a % n) + n) % n; }
 
 PwOperator::PwOperator(const PwDesc &desc) {
   n1_ = desc.n1; n2_ = desc.n2; n3_ = desc.n3;
+  if (const char *f = std::getenv("SGW_PW_FUSE")) fuse_stages_ = std::strcmp(f, "0") != 0;
   SGW_REQUIRE(n1_ > 0 && n2_ > 0 && n3_ > 0 && n1_ <= 512 && n2_ <= 512 && n3_ <= 512, SGW_E_ARG,
               "pw operator: FFT box dimensions must be in [1, 512]");
   ngroup_ = desc.ngroup;
@@ -822,20 +926,49 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
               "pw operator: prepare() was not called for this batch of vectors");
   const int nchunk = (int)((ld + kChunk - 1) / kChunk);
-  if (nproj_ > 0) {
+  const int nred_x = (nproj_ * kVT + 63) / 64;
+  const bool fused = fuse_stages_ && fft_mode_ == 0 && cubic &&
+                     (n1_ == 24 || n1_ == 32 || n1_ == 36 || n1_ == 48 || n1_ == 64);
+  if (nproj_ > 0 && !fused) {
     // projector contraction becp = D proj^H x (a concurrent stream for it was
     // measured: no gain, the plane kernel already fills the device)
     KScope ks("pw_becp", st, 2);
     k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
         d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
         part_.p);
-    k_pw_becp_reduce<<<dim3((unsigned)((nproj_ * kVT + 63) / 64), ngroup_, plan_tiles_), 64, 0,
-                       st>>>(d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
-                             part_.p, becp_.p);
+    k_pw_becp_reduce<<<dim3((unsigned)nred_x, ngroup_, plan_tiles_), 64, 0, st>>>(
+        d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk, part_.p, becp_.p);
     SGW_HIP(hipGetLastError());
   }
   auto launch_fft = [&](auto tag) {
     constexpr int NB = decltype(tag)::value;
+    if (fused) {
+      // no projectors: the part counts are zero and only the transforms run
+      const int tiles = nproj_ > 0 ? plan_tiles_ : 0;
+      const int64_t nbecp = (int64_t)nchunk * ngroup_ * tiles;
+      const int64_t nzinv = ((int64_t)gcol.x * nvec + 3) / 4;
+      const int64_t nred = (int64_t)nred_x * ngroup_ * tiles;
+      SGW_REQUIRE(nbecp + nzinv < INT32_MAX && nred + (int64_t)NB * nvec < INT32_MAX, SGW_E_ARG,
+                  "pw operator: too many vectors for one launch");
+      {
+        KScope ks("pw_becp_zinv", st, 2);
+        k_pw_becp_zinv<NB><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+            d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
+            nchunk, ngroup_, part_.p, T_.p, nvec);
+      }
+      {
+        KScope ks("pw_reduce_plane", st, 2);
+        k_pw_reduce_plane<NB><<<(unsigned)(nred + (int64_t)NB * nvec), 64, lds_fft, st>>>(
+            d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk, ngroup_,
+            nred_x, part_.p, becp_.p, T_.p, S_.p);
+      }
+      {
+        KScope ks("pw_zfwd", st, 2);
+        k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, nullptr, 0);
+      }
+      locv = loc_.p;
+      return;
+    }
     // fft_mode_: 0 FP64, 1 FP32, 2 per vector from the relax flags
     const int32_t *rl = fft_mode_ == 2 ? relax_ : nullptr;
     float2 *T32 = reinterpret_cast<float2 *>(T_.p), *S32 = reinterpret_cast<float2 *>(S_.p);
