@@ -14,7 +14,7 @@
 //
 // Layout (ld = padded vector length, zero padding is preserved by all
 // updates):  R,U: [S][l+1][ld]   X,RT: [S][ld]
-//            SU: [S][nsh][l+1][ld]   SX: [S][nsh][ld]
+//            SU: [S][nsh][ld] (u_0 of each shifted system)   SX: [S][nsh][ld]
 #include "sgw_internal.h"
 #include "bicgstab.h"
 #include "ktimer.h"
@@ -59,20 +59,14 @@ template <int NT> __device__ inline double2 block_sum(double2 v, double2 *lds) {
   return t;
 }
 
-// out[s] = sum_i a[s][i] * b[s][i]   (ZDOTU) over ld entries
-__global__ __launch_bounds__(kBlock) void k_dotu(int nsys, const int32_t *__restrict__ active,
-                                                 const double2 *__restrict__ a, int64_t sa,
-                                                 const double2 *__restrict__ b, int64_t sb,
-                                                 int64_t n, double2 *__restrict__ out) {
-  __shared__ double2 lds[kBlock / 64];
-  const int s = blockIdx.x;
-  if (!active[s]) return;
-  const double2 *pa = a + s * sa, *pb = b + s * sb;
+// sum_i a[i] * b[i]  (ZDOTU) over n entries by one kBlock block; every thread
+// returns the total
+__device__ inline double2 block_dotu(const double2 *__restrict__ pa,
+                                     const double2 *__restrict__ pb, int64_t n, double2 *lds) {
   double2 acc = mk(0, 0);
 #pragma unroll 8
   for (int64_t i = threadIdx.x; i < n; i += kBlock) cmac(acc, pa[i], pb[i]);  // loads in flight
-  acc = block_sum<kBlock>(acc, lds);
-  if (threadIdx.x == 0) out[s] = acc;
+  return block_sum<kBlock>(acc, lds);
 }
 
 // out[s] = sqrt(sum_i |a[s][i]|^2)
@@ -108,23 +102,16 @@ struct Scal {
   double2 *coef;               // [S*nsh][l+2]: gamma'_1/xi, gamma''_j/xi (j=1..l-1), 1/psi
 };
 
-// bicgstab.f90:587  rho_old = -omega * rho_old
-__global__ void k_bicg_begin(int nsys, const int32_t *__restrict__ active, Scal sc) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys || !active[s]) return;
-  sc.rho_old[s] = cmul(cneg(sc.omega[s]), sc.rho_old[s]);
-}
-
 // bicgstab.f90:595-599  rho = (r_j, r0~); beta = alpha rho / rho_old; rho_old = rho.
-// `commit` first completes L18 of the previous step (alpha_old = alpha =
-// alpha_new), which k_bicg_s2 leaves to this kernel so that its per-shift
-// threads can all read the old alpha.
+// `begin` (first step of an iteration) first applies bicgstab.f90:587,
+// rho_old = -omega rho_old.  `commit` completes L18 of the previous step
+// (alpha_old = alpha = alpha_new), which the shifted-scalar update leaves to
+// this step so that its per-shift threads can all read the old alpha.
 // `fresh` (streaming solves, first step of an iteration only): a system in its
 // first iteration has no previous step to complete; the flag is consumed here.
-__global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ active, Scal sc,
-                          int32_t *__restrict__ fresh = nullptr) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nsys || !active[s]) return;
+__device__ inline void bicg_s1_body(int s, int begin, int commit, Scal sc,
+                                    int32_t *__restrict__ fresh, double2 rho) {
+  if (begin) sc.rho_old[s] = cmul(cneg(sc.omega[s]), sc.rho_old[s]);
   if (fresh) {
     if (fresh[s]) commit = 0;
     fresh[s] = 0;
@@ -133,23 +120,34 @@ __global__ void k_bicg_s1(int nsys, int commit, const int32_t *__restrict__ acti
     sc.alpha[s] = sc.alpha_new[s];
     sc.alpha_old[s] = sc.alpha_new[s];
   }
-  double2 rho = sc.dot[s];
   sc.rho[s] = rho;
   sc.beta[s] = cdiv(cmul(sc.alpha[s], rho), sc.rho_old[s]);
   sc.rho_old[s] = rho;
 }
 
-// bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars;
-// one thread per (system, shift), every thread forms the seed alpha itself.
-// The per-step factors go to slot (s*nsh + k)*l + j: the shifted vectors are
-// updated once per iteration, for all l steps (k_shift_cycle).
-__global__ void k_bicg_s2(int nsys, int nsh, int l, int j, const int32_t *__restrict__ active,
-                          Scal sc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = t / max(nsh, 1), k = t % max(nsh, 1);
-  if (s >= nsys || !active[s]) return;
+// rho = (r_j, r0~) by block s, then the scalar step above by its thread 0
+__global__ __launch_bounds__(kBlock) void k_dotu_s1(const int32_t *__restrict__ active,
+                                                    const double2 *__restrict__ a, int64_t sa,
+                                                    const double2 *__restrict__ b, int64_t n,
+                                                    int begin, int commit, Scal sc,
+                                                    int32_t *__restrict__ fresh) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 acc = block_dotu(a + s * sa, b + s * n, n, lds);
+  if (threadIdx.x == 0) {
+    sc.dot[s] = acc;
+    bicg_s1_body(s, begin, commit, sc, fresh, acc);
+  }
+}
+
+// bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars
+// of shifted system k; `dot` = (u_{j+1}, r0~).  The per-step factors go to slot
+// (s*nsh + k)*l + j: the shifted vectors are updated once per iteration, for
+// all l steps (k_shift_cycle).
+__device__ inline void bicg_s2_body(int s, int k, int nsh, int l, int j, Scal sc, double2 dot) {
   const double2 one = mk(1, 0);
-  const double2 alpha = cdiv(sc.rho[s], sc.dot[s]);
+  const double2 alpha = cdiv(sc.rho[s], dot);
   if (k == 0) sc.alpha_new[s] = alpha;
   if (nsh == 0) return;
   const double2 beta = sc.beta[s], alpha_old = sc.alpha_old[s];
@@ -173,6 +171,20 @@ __global__ void k_bicg_s2(int nsys, int nsh, int l, int j, const int32_t *__rest
   sc.ia[pj] = cdiv(one, alpha_s);
 }
 
+// (u_{j+1}, r0~) by block s, then the shifted scalars of system s, one thread
+// per shifted system (the k = 0 thread also stores the seed alpha)
+__global__ __launch_bounds__(kBlock) void k_dotu_s2(const int32_t *__restrict__ active,
+                                                    const double2 *__restrict__ a, int64_t sa,
+                                                    const double2 *__restrict__ b, int64_t n,
+                                                    int nsh, int l, int j, Scal sc) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 acc = block_dotu(a + s * sa, b + s * n, n, lds);
+  if (threadIdx.x == 0) sc.dot[s] = acc;
+  for (int k = threadIdx.x; k < max(nsh, 1); k += kBlock) bicg_s2_body(s, k, nsh, l, j, sc, acc);
+}
+
 // ---------------------------------------------------------------------------
 // element-wise vector kernels; grid = (ceil(ld/kBlock), S)
 // ---------------------------------------------------------------------------
@@ -189,9 +201,8 @@ struct Vecs {
   __device__ double2 *r(int s, int i) const { return R + ((int64_t)s * (l + 1) + i) * ld; }
   __device__ double2 *u(int s, int i) const { return U + ((int64_t)s * (l + 1) + i) * ld; }
   __device__ double2 *x(int s) const { return X + (int64_t)s * ld; }
-  __device__ double2 *su(int s, int k, int i) const {
-    return SU + (((int64_t)s * nsh + k) * (l + 1) + i) * ld;
-  }
+  // u_0^sigma of shifted system k (u_1..u_l^sigma are rebuilt each iteration)
+  __device__ double2 *su(int s, int k) const { return SU + ((int64_t)s * nsh + k) * ld; }
   __device__ double2 *sx(int s, int k) const { return SX + ((int64_t)s * nsh + k) * ld; }
 };
 
@@ -240,8 +251,9 @@ __global__ __launch_bounds__(kBlock) void k_bicg_seed(int j, const int32_t *__re
 // vectors: the l BiCG steps (L14-L19 and L26 of Algorithm 3, with the seed
 // residual snapshots of each step) and, for systems still running after the
 // BiCG half (mr_active), the MR step (L28-L34 of Algorithm 4, with the
-// Gram-Schmidt residuals before the delayed r_0 update).  Every u_i^sigma
-// and x^sigma is read and written once per iteration.
+// Gram-Schmidt residuals before the delayed r_0 update).  u_0^sigma and
+// x^sigma are read and written once per iteration; u_1..u_l^sigma live in
+// registers only.
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restrict__ active,
                                                         const int32_t *__restrict__ mr_active,
@@ -263,9 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restric
   const int k0 = blockIdx.z * kShiftChunk, k1 = min(v.nsh, k0 + kShiftChunk);
   for (int k = k0; k < k1; ++k) {
     const int idx = s * v.nsh + k;
+    // u_1..u_l are set by L19/L26 of steps 0..l-1 before any use, so only
+    // u_0 and x carry over from the previous iteration
     double2 u[L + 1];
+    u[0] = v.su(s, k)[e];
 #pragma unroll
-    for (int i = 0; i <= l; ++i) u[i] = v.su(s, k, i)[e];
+    for (int i = 1; i <= l; ++i) u[i] = mk(0, 0);
     double2 x = v.sx(s, k)[e];
     const double2 msg = cneg(sc.sigma[idx]);
 #pragma unroll
@@ -300,8 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restric
       }
       u[0] = cmul(cf[l], u0);                                        // L34
     }
-#pragma unroll
-    for (int i = 0; i <= l; ++i) v.su(s, k, i)[e] = u[i];
+    v.su(s, k)[e] = u[0];
     v.sx(s, k)[e] = x;
   }
 }
@@ -559,7 +573,7 @@ __global__ void k_refill_vectors(const int32_t *__restrict__ pairs, Vecs v,
   v.x(s)[e] = z;
   v.RT[(int64_t)s * v.ld + e] = b;
   for (int k = 0; k < v.nsh; ++k) {
-    for (int i = 0; i <= v.l; ++i) v.su(s, k, i)[e] = z;
+    v.su(s, k)[e] = z;
     v.sx(s, k)[e] = z;
   }
 }
@@ -702,7 +716,7 @@ struct BatchBicgstab::Impl {
     X.alloc(S * ld);
     RT.alloc(S * ld);
     if (nsh > 0) {
-      SU.alloc(S * nsh * (l + 1) * ld);
+      SU.alloc(S * nsh * ld);
       SX.alloc(S * nsh * ld);
     }
     const int nsnap = snap_final(l) + 1;
@@ -772,14 +786,13 @@ struct BatchBicgstab::Impl {
     const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
     const int64_t sstride = (int64_t)(l + 1) * ld;
     // ---- bicg_part (Algorithm 3)
-    k_bicg_begin<<<grid1(S, 64), 64, 0, st>>>(S, active.p, sc);
     for (int j = 0; j < l; ++j) {
       {
         KScope ks_("dotu", st, 2);
-        k_dotu<<<S, kBlock, 0, st>>>(S, active.p, R.p + j * ld, sstride, RT.p, ld, ld, sc.dot);
+        k_dotu_s1<<<S, kBlock, 0, st>>>(active.p, R.p + j * ld, sstride, RT.p, ld, j == 0 ? 1 : 0,
+                                        (iter > 1 || j > 0) ? 1 : 0, sc,
+                                        j == 0 ? fresh : nullptr);
       }
-      k_bicg_s1<<<grid1(S, 64), 64, 0, st>>>(S, (iter > 1 || j > 0) ? 1 : 0, active.p, sc,
-                                              j == 0 ? fresh : nullptr);
       {
         KScope ks_("bicg_v1", st, 2);
         k_bicg_v1<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
@@ -787,10 +800,9 @@ struct BatchBicgstab::Impl {
       op_apply(op, st, U.p + j * ld, U.p + (j + 1) * ld);       // L10
       {
         KScope ks_("dotu", st, 2);
-        k_dotu<<<S, kBlock, 0, st>>>(S, active.p, U.p + (j + 1) * ld, sstride, RT.p, ld, ld,
-                                     sc.dot);
+        k_dotu_s2<<<S, kBlock, 0, st>>>(active.p, U.p + (j + 1) * ld, sstride, RT.p, ld, nsh, l,
+                                        j, sc);
       }
-      k_bicg_s2<<<grid1(S * std::max(nsh, 1), 64), 64, 0, st>>>(S, nsh, l, j, active.p, sc);
       {
         KScope ks_("bicg_seed", st, 2);
         k_bicg_seed<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
