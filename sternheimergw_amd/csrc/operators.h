@@ -87,6 +87,9 @@ class PwOperator : public Operator {
   // FP64 cubic path: the projector stages share launches with the z inverse
   // and plane transforms (SGW_PW_FUSE=0 in the environment keeps them apart)
   bool fuse_stages_ = true;
+  // fused path: two vectors per plane block where they fit (SGW_PW_PAIR=0
+  // keeps one)
+  bool pair_planes_ = true;
   void ensure_scratch(int nvec);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;

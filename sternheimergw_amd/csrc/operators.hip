@@ -382,6 +382,96 @@ __device__ inline void plane_body(const PwDev &d, const C *__restrict__ Tin,
   }
 }
 
+// FP64 plane transforms of two vectors per wavefront, planes (z, v0) and
+// (z, v0 + 1): the Y stages of both planes run side by side (plane p on lanes
+// [p nc1, (p + 1) nc1), which needs 2 nc1 <= 64), the X stage plane by plane on
+// lanes [0, N).  The Y stages use only nc1 lanes each, so pairing the planes
+// halves their share of the wavefront's time, and the loads of both planes are
+// in flight together.  LDS: T[2][nslot], then P[2][nc1][N].
+template <int N>
+__device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ Tin,
+                                   const int32_t *__restrict__ group,
+                                   const int32_t *__restrict__ active, double2 *__restrict__ S,
+                                   int z, int v0, int nvec, double2 *smem) {
+  const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M2 = d.M2;
+  const int lane = threadIdx.x;
+  bool live[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) live[p] = v0 + p < nvec && (!active || active[v0 + p]);
+  if (!live[0] && !live[1]) return;
+  double2 *T = smem, *P = smem + 2 * nslot;
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+    if (live[p]) stage_plane(Tin + ((int64_t)(v0 + p) * N + z) * nslot, T + p * nslot, nslot);
+  __syncthreads();
+  // Y-stage lane mapping: plane yp, row a
+  const int yp = lane < nc1 ? 0 : 1, a = lane - yp * nc1;
+  const bool yrow = a < nc1 && live[yp];
+  double2 buf[N];
+  if (yrow) {  // Y inverse of row a
+    const double2 *Tp = T + yp * nslot;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      buf[i] = (m >= -M2 && m <= M2) ? Tp[a * nc2 + m + M2] : mk(0, 0);
+    }
+    reg_fft<N, +1>(buf);
+    double2 *Pp = P + yp * nc1 * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) Pp[a * N + i] = buf[i];
+  }
+  __syncthreads();
+  if (lane < N) {  // X inverse, V(r), X forward of column y = lane, plane by plane
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      if (!live[p]) continue;
+      // Per-plane copies the compiler cannot see through: V loads, LDS
+      // addresses and band masks are then formed inside the loop.  Hoisted
+      // out of it they would stay live across both planes (2N loaded values,
+      // N addresses, 2N mask bits) and spill.
+      int y = lane, M1 = d.M1;
+      __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
+      const double *vcol = d.vloc_zxy + (int64_t)z * N * N + y;  // V(x, y, z) at vcol[x * N]
+      double2 *Pp = P + p * nc1 * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int m = i < N / 2 ? i : i - N;
+        buf[i] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N + y] : mk(0, 0);
+      }
+      reg_fft<N, +1>(buf);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double vv = vcol[i * N];
+        buf[i] = mk(buf[i].x * vv, buf[i].y * vv);
+      }
+      reg_fft<N, -1>(buf);
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int m = i < N / 2 ? i : i - N;
+        if (m >= -M1 && m <= M1) Pp[(m + M1) * N + y] = buf[i];
+      }
+    }
+  }
+  __syncthreads();
+  if (yrow) {  // Y forward of row a into the sphere columns
+    const int v = v0 + yp;
+    const double2 *Pp = P + yp * nc1 * N;
+#pragma unroll
+    for (int i = 0; i < N; ++i) buf[i] = Pp[a * N + i];
+    reg_fft<N, -1>(buf);
+    const int32_t *cc = d.col_cnt + (int64_t)(group ? group[v] : 0) * nslot;
+    double2 *Sz = S + ((int64_t)v * N + z) * nslot;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int m = i < N / 2 ? i : i - N;
+      if (m >= -M2 && m <= M2) {
+        const int c = a * nc2 + m + M2;
+        if (cc[c] > 0) Sz[c] = buf[i];
+      }
+    }
+  }
+}
+
 // Y then X inverse transforms of one z-plane of the sphere columns Tz (in
 // LDS as T) into the registers of lane y: buf[x] = value at (x, y, z).
 // P (nc1 x N, LDS) is the row buffer; lanes >= N return with buf untouched.
@@ -721,12 +811,14 @@ __global__ __launch_bounds__(256) void k_pw_becp_zinv(
                         (zb % gx) * 64 + (int)(threadIdx.x & 63));
 }
 
-template <int N>
+// PAIR: two vectors per plane block (plane2_body), else one (plane_body)
+template <int N, bool PAIR>
 __global__ __launch_bounds__(64) void k_pw_reduce_plane(
     PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
     int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
-    double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S) {
+    double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S,
+    int nvec) {
   extern __shared__ double2 smem_d[];
   const int nred = nred_x * ngroup * tiles, b = blockIdx.x;
   if (b < nred) {
@@ -735,7 +827,10 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
     return;
   }
   const int pb = b - nred;
-  plane_body<N, double2>(d, T, group, active, S, nullptr, 0, pb % N, pb / N, smem_d);
+  if constexpr (PAIR)
+    plane2_body<N>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
+  else
+    plane_body<N, double2>(d, T, group, active, S, nullptr, 0, pb % N, pb / N, smem_d);
 }
 
 }  // namespace
@@ -799,6 +894,7 @@ static inline int modn(int a, int n) { return ((a % n) + n) % n; }
 PwOperator::PwOperator(const PwDesc &desc) {
   n1_ = desc.n1; n2_ = desc.n2; n3_ = desc.n3;
   if (const char *f = std::getenv("SGW_PW_FUSE")) fuse_stages_ = std::strcmp(f, "0") != 0;
+  if (const char *f = std::getenv("SGW_PW_PAIR")) pair_planes_ = std::strcmp(f, "0") != 0;
   SGW_REQUIRE(n1_ > 0 && n2_ > 0 && n3_ > 0 && n1_ <= 512 && n2_ <= 512 && n3_ <= 512, SGW_E_ARG,
               "pw operator: FFT box dimensions must be in [1, 512]");
   ngroup_ = desc.ngroup;
@@ -958,9 +1054,23 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       }
       {
         KScope ks("pw_reduce_plane", st, 2);
-        k_pw_reduce_plane<NB><<<(unsigned)(nred + (int64_t)NB * nvec), 64, lds_fft, st>>>(
-            d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk, ngroup_,
-            nred_x, part_.p, becp_.p, T_.p, S_.p);
+        // two vectors per plane block while both Y stages fit in one wavefront
+        // and four such blocks still fit in a CU's LDS (N = 64 would spill)
+        bool paired = false;
+        if constexpr (NB <= 48) {
+          if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
+            k_pw_reduce_plane<NB, true>
+                <<<(unsigned)(nred + (int64_t)NB * ((nvec + 1) / 2)), 64, 2 * lds_fft, st>>>(
+                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
+                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+            paired = true;
+          }
+        }
+        if (!paired) {
+          k_pw_reduce_plane<NB, false><<<(unsigned)(nred + (int64_t)NB * nvec), 64, lds_fft, st>>>(
+              d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk, ngroup_,
+              nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+        }
       }
       {
         KScope ks("pw_zfwd", st, 2);
