@@ -64,6 +64,35 @@ def test_pw_apply_multigroup(multi_q):
         assert np.linalg.norm(ax - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
+@pytest.mark.parametrize("env", ["SGW_PW_PAIR", "SGW_PW_FUSE"])
+def test_pw_apply_launch_layouts_agree(multi_q, monkeypatch, env):
+    """The FP64 cubic path's launch layouts: the default (projector stages
+    sharing launches with the transforms, two vectors per plane block) and,
+    with `env`=0, one vector per plane block or separate kernels per stage.
+    An odd number of vectors over three groups leaves a half-filled last
+    pair.  Both layouts match the oracle to 1e-12 and each other to 1e-14
+    (they may round differently where FMAs are formed differently)."""
+    rng = _rng(9)
+    nv = 11
+    grp = np.arange(nv) % len(multi_q.spheres)
+    ld = max(multi_q.npw)
+    x = rng.standard_normal((nv, ld)) + 1j * rng.standard_normal((nv, ld))
+    for v in range(nv):
+        x[v, multi_q.npw[grp[v]]:] = 0
+    sig = 0.1 - 0.7j
+    out = {}
+    for val in ("1", "0"):
+        monkeypatch.setenv(env, val)
+        out[val] = operators.PwOperator.from_workload(multi_q).apply(x, sig, group=grp)
+    for v in range(nv):
+        n = multi_q.npw[grp[v]]
+        ref = orc.PwOp(multi_q, int(grp[v])).apply(x[v, :n], sig)
+        scale = np.linalg.norm(ref)
+        for val in ("1", "0"):
+            assert np.linalg.norm(out[val][v, :n] - ref) <= 1e-12 * scale, (val, v)
+        assert np.linalg.norm(out["1"][v] - out["0"][v]) <= 1e-14 * scale, v
+
+
 def test_pw_operator_hermitian(c1):
     """<x, H y> = <H x, y> for the unshifted operator (V real, D real)."""
     op = operators.PwOperator.from_workload(c1)
