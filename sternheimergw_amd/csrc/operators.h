@@ -90,6 +90,9 @@ class PwOperator : public Operator {
   // fused path: two vectors per plane block where they fit (SGW_PW_PAIR=0
   // keeps one)
   bool pair_planes_ = true;
+  // paired planes with the transforms split over lane pairs (SGW_PW_SPLIT=0
+  // keeps one lane per row or column)
+  bool split_planes_ = true;
   void ensure_scratch(int nvec);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;

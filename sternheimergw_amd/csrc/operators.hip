@@ -13,6 +13,7 @@
 #include "operators.h"
 #include "ktimer.h"
 #include "regfft.h"
+#include "splitfft.h"
 
 #include <algorithm>
 #include <cmath>
@@ -472,6 +473,109 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
   }
 }
 
+// FP64 plane transforms of two vectors, planes (z, v0) and (z, v0 + 1), by a
+// block of 128 lanes whose transforms are split over lane pairs (splitfft.h):
+// every lane holds N/2 values, so the block needs about half the registers
+// per lane of plane2_body and the SIMDs hold three wavefronts instead of one.
+//   Y  lane pair r < 2 nc1: row a of plane yp (r = yp nc1 + a), m2 -> y;
+//   X  lane pair r < N: column y = r of each live plane in turn, m1 -> x,
+//      times V(x, y, z), x -> m1 (the nc1 sphere frequencies kept);
+//   Y' as Y, y -> m2, into the sphere columns S[v][z][c].
+// LDS holds P[2][nc1][N]; the column values T[2][nslot] are staged into its
+// start and read into registers before the Y stage overwrites them.
+template <int N>
+__device__ inline void plane_split_body(const PwDev &d, const double2 *__restrict__ Tin,
+                                        const int32_t *__restrict__ group,
+                                        const int32_t *__restrict__ active,
+                                        double2 *__restrict__ S, int z, int v0, int nvec,
+                                        double2 *smem) {
+  constexpr int H = N / 2, Q = N / 4;
+  const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M2 = d.M2;
+  const int t = threadIdx.x, r = t >> 1;
+  const bool odd = (t & 1) != 0;
+  bool live[2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) live[p] = v0 + p < nvec && (!active || active[v0 + p]);
+  if (!live[0] && !live[1]) return;
+  double2 *P = smem, *T = smem;  // T[2][nslot] aliases the start of P[2][nc1][N]
+  if (t < 64) {  // one wavefront stages both planes' columns
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+      if (live[p]) stage_plane(Tin + ((int64_t)(v0 + p) * N + z) * nslot, T + p * nslot, nslot);
+  }
+  __syncthreads();
+  const int yp = r < nc1 ? 0 : 1, a = r - yp * nc1;
+  const bool yrow = a < nc1 && live[yp];
+  double2 b[H];
+  if (yrow) {
+    const double2 *Tp = T + yp * nslot + a * nc2 + M2;
+#pragma unroll
+    for (int n = 0; n < H; ++n) {
+      const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+      b[n] = (m >= -M2 && m <= M2) ? Tp[m] : mk(0, 0);
+    }
+  }
+  __syncthreads();  // every column value is in registers before P overwrites T
+  if (yrow) {  // Y inverse of row a
+    split_dit<N, +1>(b, odd);
+    double2 *Pr = P + (yp * nc1 + a) * N + (odd ? Q : 0);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      Pr[j] = b[j];
+      Pr[H + j] = b[Q + j];
+    }
+  }
+  __syncthreads();
+  if (r < N) {  // X inverse, V(r), X forward of column y = r, plane by plane
+#pragma unroll 1
+    for (int p = 0; p < 2; ++p) {
+      if (!live[p]) continue;
+      // per-plane copies the compiler cannot see through, so that addresses
+      // and masks are formed inside the loop rather than kept live across it
+      int y = r, M1 = d.M1;
+      __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
+      double2 *Pp = P + p * nc1 * N + y;
+#pragma unroll
+      for (int n = 0; n < H; ++n) {
+        const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+        b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N] : mk(0, 0);
+      }
+      split_dit<N, +1>(b, odd);
+      const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) {  // x = j (+Q) and H + j (+Q)
+        const double v1 = vcol[j * N], v2 = vcol[(H + j) * N];
+        b[j] = mk(b[j].x * v1, b[j].y * v1);
+        b[Q + j] = mk(b[Q + j].x * v2, b[Q + j].y * v2);
+      }
+      split_dif<N, -1>(b, odd);
+#pragma unroll
+      for (int k = 0; k < H; ++k) {
+        const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+        if (m >= -M1 && m <= M1) Pp[(m + M1) * N] = b[k];
+      }
+    }
+  }
+  __syncthreads();
+  if (yrow) {  // Y forward of row a into the sphere columns
+    const int v = v0 + yp;
+    const double2 *Pr = P + (yp * nc1 + a) * N + (odd ? Q : 0);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      b[j] = Pr[j];
+      b[Q + j] = Pr[H + j];
+    }
+    split_dif<N, -1>(b, odd);
+    const int32_t *cc = d.col_cnt + (int64_t)(group ? group[v] : 0) * nslot + a * nc2 + M2;
+    double2 *Sz = S + ((int64_t)v * N + z) * nslot + a * nc2 + M2;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+      if (m >= -M2 && m <= M2 && cc[m] > 0) Sz[m] = b[k];
+    }
+  }
+}
+
 // Y then X inverse transforms of one z-plane of the sphere columns Tz (in
 // LDS as T) into the registers of lane y: buf[x] = value at (x, y, z).
 // P (nc1 x N, LDS) is the row buffer; lanes >= N return with buf untouched.
@@ -603,7 +707,7 @@ __device__ inline void becp_reduce_body(const PwDev &d, const int32_t *__restric
                                         const int32_t *__restrict__ gvec,
                                         const int32_t *__restrict__ gcnt, int maxper, int tiles,
                                         int nchunk, const double2 *__restrict__ part,
-                                        double2 *__restrict__ becp, int bx, int g, int vt);
+                                        double2 *__restrict__ becp, int bx, int g, int vt, int t);
 
 __global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
                                                        const int32_t *__restrict__ active,
@@ -613,18 +717,18 @@ __global__ __launch_bounds__(64) void k_pw_becp_reduce(PwDev d,
                                                        const double2 *__restrict__ part,
                                                        double2 *__restrict__ becp) {
   becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, blockIdx.x,
-                   blockIdx.y, blockIdx.z);
+                   blockIdx.y, blockIdx.z, threadIdx.x);
 }
 
 __device__ inline void becp_reduce_body(const PwDev &d, const int32_t *__restrict__ active,
                                         const int32_t *__restrict__ gvec,
                                         const int32_t *__restrict__ gcnt, int maxper, int tiles,
                                         int nchunk, const double2 *__restrict__ part,
-                                        double2 *__restrict__ becp, int bx, int g, int vt) {
+                                        double2 *__restrict__ becp, int bx, int g, int vt, int t) {
   const int cnt = gcnt[g], v0 = vt * kVT;
   if (v0 >= cnt) return;
   const int nv = min(kVT, cnt - v0), nproj = d.nproj;
-  const int o = bx * 64 + threadIdx.x;
+  const int o = bx * 64 + t;  // t: thread of the 64 handling outputs [64 bx, 64 bx + 64)
   if (o >= nproj * kVT) return;
   if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   const int nck = (d.npw[g] + kChunk - 1) / kChunk;
@@ -823,7 +927,7 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
   const int nred = nred_x * ngroup * tiles, b = blockIdx.x;
   if (b < nred) {
     becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, b % nred_x,
-                     (b / nred_x) % ngroup, b / (nred_x * ngroup));
+                     (b / nred_x) % ngroup, b / (nred_x * ngroup), threadIdx.x);
     return;
   }
   const int pb = b - nred;
@@ -831,6 +935,28 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
     plane2_body<N>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
   else
     plane_body<N, double2>(d, T, group, active, S, nullptr, 0, pb % N, pb / N, smem_d);
+}
+
+// As k_pw_reduce_plane with the split plane transforms (128-lane blocks):
+// a reduction block covers two of the 64-output ranges.
+template <int N>
+__global__ __launch_bounds__(128) void k_pw_reduce_split(
+    PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
+    const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
+    int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
+    double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S,
+    int nvec) {
+  extern __shared__ double2 smem_d[];
+  const int nred_h = (nred_x + 1) / 2, nred = nred_h * ngroup * tiles, b = blockIdx.x;
+  if (b < nred) {
+    const int bx = 2 * (b % nred_h) + (int)(threadIdx.x >> 6);
+    if (bx < nred_x)
+      becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, bx,
+                       (b / nred_h) % ngroup, b / (nred_h * ngroup), threadIdx.x & 63);
+    return;
+  }
+  const int pb = b - nred;
+  plane_split_body<N>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
 }
 
 }  // namespace
@@ -895,6 +1021,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
   n1_ = desc.n1; n2_ = desc.n2; n3_ = desc.n3;
   if (const char *f = std::getenv("SGW_PW_FUSE")) fuse_stages_ = std::strcmp(f, "0") != 0;
   if (const char *f = std::getenv("SGW_PW_PAIR")) pair_planes_ = std::strcmp(f, "0") != 0;
+  if (const char *f = std::getenv("SGW_PW_SPLIT")) split_planes_ = std::strcmp(f, "0") != 0;
   SGW_REQUIRE(n1_ > 0 && n2_ > 0 && n3_ > 0 && n1_ <= 512 && n2_ <= 512 && n3_ <= 512, SGW_E_ARG,
               "pw operator: FFT box dimensions must be in [1, 512]");
   ngroup_ = desc.ngroup;
@@ -1058,7 +1185,16 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
         // and four such blocks still fit in a CU's LDS (N = 64 would spill)
         bool paired = false;
         if constexpr (NB <= 48) {
-          if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
+          // split transforms over lane pairs (LDS: P of both planes only)
+          const size_t lds_split = sizeof(double2) * 2 * (size_t)nc1_ * NB;
+          if (split_planes_ && pair_planes_ && 2 * nc1_ <= 64 && lds_split <= 64 * 1024) {
+            const int64_t nred_s = (int64_t)((nred_x + 1) / 2) * ngroup_ * tiles;
+            k_pw_reduce_split<NB>
+                <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + 1) / 2)), 128, lds_split, st>>>(
+                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
+                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+            paired = true;
+          } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>
                 <<<(unsigned)(nred + (int64_t)NB * ((nvec + 1) / 2)), 64, 2 * lds_fft, st>>>(
                     d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,

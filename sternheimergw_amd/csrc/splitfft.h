@@ -1,0 +1,90 @@
+// Length-N FFTs split over a lane pair (lanes 2r and 2r+1 of a wavefront),
+// each lane holding N/2 values: one radix-2 step across the pair, exchanged
+// with a DPP lane swap, and one register FFT of length N/2 per lane.  Halving
+// the values per lane halves the registers a transform needs, so a kernel
+// built on these runs more wavefronts per SIMD than one on reg_fft<N>.
+//
+// Layouts (Q = N/4, H = N/2, `odd` = lane index & 1):
+//   split_dit: in  b[n] = x[2n + odd]                      (decimated)
+//              out b[j] = X[j + Q odd], b[Q + j] = X[H + j + Q odd]
+//   split_dif: in  b[j] = X[j + Q odd], b[Q + j] = X[H + j + Q odd]
+//              out b[k] = F[2k + odd]                      (decimated)
+// with X (F) the transform of x (X) in the direction SIGN as in reg_fft:
+// sum_n x[n] e^{SIGN 2 pi i n k / N}, unscaled.  Both lanes of a pair must
+// execute these together (the swap reads the partner's registers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "regfft.h"
+
+namespace sgw {
+
+// the partner lane's value (lane ^ 1): DPP quad permutation [1, 0, 3, 2]
+__device__ inline double lane_swap(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xB1, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double2 lane_swap(double2 v) {
+  return make_double2(lane_swap(v.x), lane_swap(v.y));
+}
+__device__ inline double2 pick(bool c, double2 a, double2 b) {
+  return make_double2(c ? a.x : b.x, c ? a.y : b.y);
+}
+
+namespace detail {
+// v[k] *= e^{SIGN 2 pi i k / N}, k in the sequence
+template <int N, int SIGN, int... K>
+__device__ inline void twiddle_seq(double2 *v, std::integer_sequence<int, K...>) {
+  ((v[K] = twiddle<K, N, N, SIGN>(v[K])), ...);
+}
+}  // namespace detail
+
+template <int N, int SIGN> __device__ inline void split_dit(double2 (&b)[N / 2], bool odd) {
+  static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
+  constexpr int H = N / 2, Q = N / 4;
+  double2 y[H];
+  RegFft<H, N, SIGN, double2>::run(b, y);  // E (even lane) or O (odd lane)
+  if (odd) detail::twiddle_seq<N, SIGN>(y, std::make_integer_sequence<int, H>{});
+  // even lane: e = E[j], o = O'[j] from the partner -> X[j], X[H + j]
+  // odd lane:  e = E[Q + j] from the partner, o = O'[Q + j] -> X[Q + j], X[H + Q + j]
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const double2 recv = lane_swap(pick(odd, y[j], y[Q + j]));
+    const double2 e = pick(odd, recv, y[j]), o = pick(odd, y[Q + j], recv);
+    b[j] = rf_add(e, o);
+    b[Q + j] = rf_sub(e, o);
+  }
+}
+
+template <int N, int SIGN> __device__ inline void split_dif(double2 (&b)[N / 2], bool odd) {
+  static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
+  constexpr int H = N / 2, Q = N / 4;
+  double2 a[Q], d[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    a[j] = rf_add(b[j], b[Q + j]);
+    d[j] = rf_sub(b[j], b[Q + j]);
+  }
+  // d *= W^(j + Q odd), W^Q = SIGN i
+  detail::twiddle_seq<N, SIGN>(d, std::make_integer_sequence<int, Q>{});
+  if (odd) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) d[j] = rf_muli<SIGN>(d[j]);
+  }
+  // even lane keeps a over all x (its half, then the partner's), odd lane d
+  double2 x[H];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const double2 recv = lane_swap(pick(odd, a[j], d[j]));
+    x[j] = pick(odd, recv, a[j]);
+    x[Q + j] = pick(odd, d[j], recv);
+  }
+  RegFft<H, N, SIGN, double2>::run(x, b);
+}
+
+}  // namespace sgw
