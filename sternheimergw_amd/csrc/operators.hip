@@ -286,6 +286,80 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
   }
 }
 
+// FP64 z transforms with each column split over a lane pair (splitfft.h):
+// lane 2c + odd of the pair of column c holds N/2 of its values, so these
+// need about half the registers of k_pw_zinv / k_pw_zfwd.  Both lanes of a
+// pair take the same early exits.
+template <int N>
+__device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict__ in,
+                                       int64_t sin, const int32_t *__restrict__ group,
+                                       const int32_t *__restrict__ active,
+                                       double2 *__restrict__ T, int v, int c, bool odd) {
+  constexpr int H = N / 2, Q = N / 4;
+  if (active && !active[v]) return;
+  if (c >= d.nslot) return;
+  const int g = group ? group[v] : 0;
+  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  double2 *Tv = T + (int64_t)v * N * d.nslot + c;
+  const int zo = odd ? Q : 0;  // this lane's z: zo + j and H + zo + j
+  if (cnt == 0) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      Tv[(int64_t)(zo + j) * d.nslot] = mk(0, 0);
+      Tv[(int64_t)(H + zo + j) * d.nslot] = mk(0, 0);
+    }
+    return;
+  }
+  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
+  const double2 *x = in + v * sin;
+  double2 b[H];
+#pragma unroll
+  for (int n = 0; n < H; ++n) {
+    const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k = m - lo;
+    b[n] = (k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
+  }
+  split_dit<N, +1>(b, odd);
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    Tv[(int64_t)(zo + j) * d.nslot] = b[j];
+    Tv[(int64_t)(H + zo + j) * d.nslot] = b[Q + j];
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const double2 *__restrict__ S,
+                                                      const int32_t *__restrict__ group,
+                                                      const int32_t *__restrict__ active,
+                                                      double2 *__restrict__ loc) {
+  constexpr int H = N / 2, Q = N / 4;
+  const int v = blockIdx.y, c = blockIdx.x * 32 + (int)(threadIdx.x >> 1);
+  const bool odd = (threadIdx.x & 1) != 0;
+  if (active && !active[v]) return;
+  if (c >= d.nslot) return;
+  const int g = group ? group[v] : 0;
+  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  if (cnt == 0) return;
+  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
+  const double2 *Sv = S + (int64_t)v * N * d.nslot + c;
+  const int zo = odd ? Q : 0;
+  double2 b[H];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    b[j] = Sv[(int64_t)(zo + j) * d.nslot];
+    b[Q + j] = Sv[(int64_t)(H + zo + j) * d.nslot];
+  }
+  split_dif<N, -1>(b, odd);
+  double2 *lv = loc + (int64_t)v * d.ld;
+  const double sc = d.inv_nbox;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k3 = m - lo;
+    if (k3 >= 0 && k3 < cnt) lv[sidx[k3]] = mk(b[k].x * sc, b[k].y * sc);
+  }
+}
+
 // One wavefront per (plane z, vector v), cubic box of compile-time side N.
 // The plane's pencil transforms run as register FFTs (regfft.h):
 //   Z  each lane sums its sphere columns at this z (<= 2M3+1 terms),
@@ -896,7 +970,9 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
 //                      rest are the plane transforms (z fastest).
 // The block index orders match the separate kernels' grids, and every branch
 // on it is uniform within a block, so the barriers of either part are safe.
-template <int N>
+// SPLIT: the z inverse transforms split over lane pairs (32 columns per
+// 64-lane group), else one lane per column.
+template <int N, bool SPLIT>
 __global__ __launch_bounds__(256) void k_pw_becp_zinv(
     PwDev d, const double2 *__restrict__ in, int64_t sin, const int32_t *__restrict__ group,
     const int32_t *__restrict__ active, const int32_t *__restrict__ gvec,
@@ -908,11 +984,17 @@ __global__ __launch_bounds__(256) void k_pw_becp_zinv(
               (b / nchunk) % ngroup, b / (nchunk * ngroup));
     return;
   }
-  const int gx = (d.nslot + 63) / 64;
+  constexpr int kCols = SPLIT ? 32 : 64;  // columns per 64-lane group
+  const int gx = (d.nslot + kCols - 1) / kCols;
   const int zb = (b - nbecp) * 4 + (int)(threadIdx.x >> 6);
   if (zb >= gx * nvec) return;
-  zinv_body<N, double2>(d, in, sin, group, active, T, nullptr, 0, zb / gx,
-                        (zb % gx) * 64 + (int)(threadIdx.x & 63));
+  const int lane = (int)(threadIdx.x & 63);
+  if constexpr (SPLIT)
+    zinv_split_body<N>(d, in, sin, group, active, T, zb / gx, (zb % gx) * 32 + (lane >> 1),
+                       (lane & 1) != 0);
+  else
+    zinv_body<N, double2>(d, in, sin, group, active, T, nullptr, 0, zb / gx,
+                          (zb % gx) * 64 + lane);
 }
 
 // PAIR: two vectors per plane block (plane2_body), else one (plane_body)
@@ -1169,15 +1251,23 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       // no projectors: the part counts are zero and only the transforms run
       const int tiles = nproj_ > 0 ? plan_tiles_ : 0;
       const int64_t nbecp = (int64_t)nchunk * ngroup_ * tiles;
-      const int64_t nzinv = ((int64_t)gcol.x * nvec + 3) / 4;
+      // z transforms split over lane pairs: 32 columns per 64 lanes
+      const bool zsplit = split_planes_;
+      const int64_t gxz = zsplit ? (nslot_ + 31) / 32 : (int64_t)gcol.x;
+      const int64_t nzinv = (gxz * nvec + 3) / 4;
       const int64_t nred = (int64_t)nred_x * ngroup_ * tiles;
       SGW_REQUIRE(nbecp + nzinv < INT32_MAX && nred + (int64_t)NB * nvec < INT32_MAX, SGW_E_ARG,
                   "pw operator: too many vectors for one launch");
       {
         KScope ks("pw_becp_zinv", st, 2);
-        k_pw_becp_zinv<NB><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
-            d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
-            nchunk, ngroup_, part_.p, T_.p, nvec);
+        if (zsplit)
+          k_pw_becp_zinv<NB, true><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+              d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
+              nchunk, ngroup_, part_.p, T_.p, nvec);
+        else
+          k_pw_becp_zinv<NB, false><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+              d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
+              nchunk, ngroup_, part_.p, T_.p, nvec);
       }
       {
         KScope ks("pw_reduce_plane", st, 2);
@@ -1210,7 +1300,11 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       }
       {
         KScope ks("pw_zfwd", st, 2);
-        k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, nullptr, 0);
+        if (zsplit)
+          k_pw_zfwd_split<NB><<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, S_.p, d_group,
+                                                                         d_active, loc_.p);
+        else
+          k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, nullptr, 0);
       }
       locv = loc_.p;
       return;
