@@ -29,7 +29,7 @@ namespace sgw {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kShiftChunk = 8;  // default shifted systems per block of k_shift_cycle
+constexpr int kShiftGroups = 4;  // default shifted-system groups per k_shift_cycle block
 
 // ---------------------------------------------------------------------------
 // reductions
@@ -258,10 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_bicg_seed(int j, const int32_t *__re
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restrict__ active,
                                                         const int32_t *__restrict__ mr_active,
-                                                        Vecs v, Scal sc, int chunk) {
-  const int s = blockIdx.y;
+                                                        Vecs v, Scal sc, int groups) {
+  // block = (kBlock / groups elements) x (groups of shifted systems): the
+  // groups of one element read the same snapshots, which reach HBM once
+  const int s = blockIdx.y, ne = kBlock / groups;
   if (!active[s]) return;
-  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int kg = threadIdx.x / ne;
+  const int64_t e = (int64_t)blockIdx.x * ne + threadIdx.x % ne;
   if (e >= v.ld) return;
   constexpr int l = L;  // compile-time degree: every array below stays in registers
   constexpr int nsnap = L * (L + 1) / 2 + 1;
@@ -273,8 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restric
 #pragma unroll
   for (int i = 0; i <= l; ++i) rr[i] = do_mr ? v.r(s, i)[e] : mk(0, 0);
   const double2 *gam = sc.gamma + s * l;
-  const int k0 = blockIdx.z * chunk, k1 = min(v.nsh, k0 + chunk);
-  for (int k = k0; k < k1; ++k) {
+  for (int k = kg; k < v.nsh; k += groups) {
     const int idx = s * v.nsh + k;
     // u_1..u_l are set by L19/L26 of steps 0..l-1 before any use, so only
     // u_0 and x carry over from the previous iteration
@@ -682,7 +684,7 @@ template <int L> struct MrSeed {
 
 struct BatchBicgstab::Impl {
   int nsys, nshift, nsh, l;
-  int shift_chunk = kShiftChunk;  // shifted systems per k_shift_cycle block
+  int shift_groups = kShiftGroups;  // shifted-system groups per k_shift_cycle block
   int64_t ld;
   DevBuf<double2> R, U, X, RT, SU, SX, RS;
   DevBuf<double2> scal_seed, scal_shift, mu, coef, gam;
@@ -711,7 +713,10 @@ struct BatchBicgstab::Impl {
   Impl(int nsys_, int nshift_, int l_, int64_t ld_)
       : nsys(nsys_), nshift(nshift_), nsh(nshift_ - 1), l(l_), ld(ld_) {
     SGW_REQUIRE(l >= 1 && l <= kMaxL, SGW_E_ARG, "bicgstab: lmax must be in [1, 15]");
-    if (const char *c = std::getenv("SGW_SHIFT_CHUNK")) shift_chunk = std::max(1, std::atoi(c));
+    if (const char *c = std::getenv("SGW_SHIFT_GROUPS")) {  // 1, 2, 4, 8 or 16
+      const int g = std::atoi(c);
+      if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16) shift_groups = g;
+    }
     SGW_REQUIRE(nshift >= 1, SGW_E_ARG, "bicgstab: need at least one shift");
     const size_t S = nsys;
     R.alloc(S * (l + 1) * ld);
@@ -820,9 +825,10 @@ struct BatchBicgstab::Impl {
     }
     if (nsh > 0) {  // the shifted systems: BiCG half of this iteration and, if running, MR half
       KScope ks_("shift_cycle", st, 2);
-      const dim3 gs(gv.x, gv.y, (unsigned)((nsh + shift_chunk - 1) / shift_chunk));
+      const int ne = kBlock / shift_groups;
+      const dim3 gs((unsigned)((ld + ne - 1) / ne), S);
       launch_l<ShiftCycle>(l, gs, dim3(kBlock), st, (const int32_t *)active.p,
-                           (const int32_t *)mr_active.p, vv, sc, shift_chunk);
+                           (const int32_t *)mr_active.p, vv, sc, shift_groups);
     }
     {
       KScope ks_("mr_seed", st, 2);
