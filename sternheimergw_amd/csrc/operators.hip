@@ -591,7 +591,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
   }
   __syncthreads();  // every column value is in registers before P overwrites T
   if (yrow) {  // Y inverse of row a
-    split_dit<N, +1>(b, odd);
+    split_dit_sel<N, +1>(b, odd);
     double2 *Pr = P + (yp * nc1 + a) * N + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
@@ -614,7 +614,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
         const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
         b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N] : mk(0, 0);
       }
-      split_dit<N, +1>(b, odd);
+      split_dit_sel<N, +1>(b, odd);
       const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
 #pragma unroll
       for (int j = 0; j < Q; ++j) {  // x = j (+Q) and H + j (+Q)
@@ -622,7 +622,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
         b[j] = mk(b[j].x * v1, b[j].y * v1);
         b[Q + j] = mk(b[Q + j].x * v2, b[Q + j].y * v2);
       }
-      split_dif<N, -1>(b, odd);
+      split_dif_sel<N, -1>(b, odd);
 #pragma unroll
       for (int k = 0; k < H; ++k) {
         const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
@@ -639,7 +639,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
       b[j] = Pr[j];
       b[Q + j] = Pr[H + j];
     }
-    split_dif<N, -1>(b, odd);
+    split_dif_sel<N, -1>(b, odd);
     const int32_t *cc = d.col_cnt + (int64_t)(group ? group[v] : 0) * nslot + a * nc2 + M2;
     double2 *Sz = S + ((int64_t)v * N + z) * nslot + a * nc2 + M2;
 #pragma unroll

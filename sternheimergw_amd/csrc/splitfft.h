@@ -32,6 +32,69 @@ __device__ inline double lane_swap(double v) {
 __device__ inline double2 lane_swap(double2 v) {
   return make_double2(lane_swap(v.x), lane_swap(v.y));
 }
+__device__ inline double2 neg(double2 v) { return make_double2(-v.x, -v.y); }
+
+namespace detail {
+// v[k] *= e^{SIGN 2 pi i ((k + Q) mod H) / N}, k in the sequence
+template <int N, int SIGN, int... K>
+__device__ inline void twiddle_rot(double2 *v, std::integer_sequence<int, K...>) {
+  constexpr int H = N / 2, Q = N / 4;
+  ((v[K] = twiddle<(K + Q) % H, N, N, SIGN>(v[K])), ...);
+}
+}  // namespace detail
+
+// The odd lane works on its half rotated by Q = H/2 positions, which makes
+// the register it sends and the one it keeps the same as the even lane's: no
+// per-lane selects around the swap.  A rotation by H/2 of a length-H
+// transform's input or output is a factor (-1)^n on the other side.
+template <int N, int SIGN> __device__ inline void split_dit(double2 (&b)[N / 2], bool odd) {
+  static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
+  constexpr int H = N / 2, Q = N / 4;
+  if (odd) {  // (-1)^n: the odd lane's transform comes out rotated by Q
+#pragma unroll
+    for (int n = 1; n < H; n += 2) b[n] = neg(b[n]);
+  }
+  double2 y[H];
+  RegFft<H, N, SIGN, double2>::run(b, y);  // even: E[k]; odd: O[(k + Q) mod H]
+  if (odd) detail::twiddle_rot<N, SIGN>(y, std::make_integer_sequence<int, H>{});
+  // even lane: y[j] = E[j], receives O'[j]          -> X[j], X[H + j]
+  // odd lane:  y[j] = O'[Q + j], receives E[Q + j]  -> X[Q + j], -X[H + Q + j]
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const double2 recv = lane_swap(y[Q + j]);
+    b[j] = rf_add(y[j], recv);
+    b[Q + j] = rf_sub(y[j], recv);
+  }
+  if (odd) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) b[Q + j] = neg(b[Q + j]);
+  }
+}
+
+template <int N, int SIGN> __device__ inline void split_dif(double2 (&b)[N / 2], bool odd) {
+  static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
+  constexpr int H = N / 2, Q = N / 4;
+  if (odd) {  // the odd lane forms (difference, sum) where the even lane forms (sum, difference)
+#pragma unroll
+    for (int j = 0; j < Q; ++j) b[Q + j] = neg(b[Q + j]);
+  }
+  // even lane: x = the sums over all x (its own, then the partner's);
+  // odd lane: the differences rotated by Q (its own at x = Q + j, then the
+  // partner's at x = j), twiddled by W^x
+  double2 x[H];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    x[j] = rf_add(b[j], b[Q + j]);
+    x[Q + j] = lane_swap(rf_sub(b[j], b[Q + j]));
+  }
+  if (odd) detail::twiddle_rot<N, SIGN>(x, std::make_integer_sequence<int, H>{});
+  RegFft<H, N, SIGN, double2>::run(x, b);
+  if (odd) {  // undo the rotation: (-1)^k
+#pragma unroll
+    for (int k = 1; k < H; k += 2) b[k] = neg(b[k]);
+  }
+}
+
 __device__ inline double2 pick(bool c, double2 a, double2 b) {
   return make_double2(c ? a.x : b.x, c ? a.y : b.y);
 }
@@ -44,7 +107,9 @@ __device__ inline void twiddle_seq(double2 *v, std::integer_sequence<int, K...>)
 }
 }  // namespace detail
 
-template <int N, int SIGN> __device__ inline void split_dit(double2 (&b)[N / 2], bool odd) {
+// Select-based forms: each lane picks what it sends and keeps (per-lane
+// selects around the swap, no rotation).  Same layouts and results.
+template <int N, int SIGN> __device__ inline void split_dit_sel(double2 (&b)[N / 2], bool odd) {
   static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
   constexpr int H = N / 2, Q = N / 4;
   double2 y[H];
@@ -61,7 +126,7 @@ template <int N, int SIGN> __device__ inline void split_dit(double2 (&b)[N / 2],
   }
 }
 
-template <int N, int SIGN> __device__ inline void split_dif(double2 (&b)[N / 2], bool odd) {
+template <int N, int SIGN> __device__ inline void split_dif_sel(double2 (&b)[N / 2], bool odd) {
   static_assert(N % 4 == 0, "split FFT needs N divisible by 4");
   constexpr int H = N / 2, Q = N / 4;
   double2 a[Q], d[Q];
