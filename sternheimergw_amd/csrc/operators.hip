@@ -885,6 +885,9 @@ __device__ inline void becp_body(const PwDev &d, const double2 *__restrict__ in,
 // out_v = kin x_v + fwfft(V invfft x_v) + proj (D becp_v) + sigma_v x_v for the
 // plane waves of one chunk and the vectors of one tile; becp_v is the sum of
 // the chunk partials, the local term the z forward transform of S.
+// CH plane waves per block: thread = (plane wave i < CH, vector group vq <
+// 256 / CH), each thread kVT / (256 / CH) vectors.
+template <int CH>
 __global__ __launch_bounds__(256) void k_pw_assemble2(
     PwDev d, const double2 *__restrict__ in, int64_t sin, double2 *__restrict__ out, int64_t sout,
     const double2 *__restrict__ sigma, const int32_t *__restrict__ active,
@@ -896,7 +899,7 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   const int cnt = gcnt[g], v0 = vt * kVT;
   if (v0 >= cnt) return;
   const int nv = min(kVT, cnt - v0), nproj = d.nproj, n3 = d.n3;
-  const int npw = d.npw[g], i0 = chunk * kChunk;
+  const int npw = d.npw[g], i0 = chunk * CH;
   if (active && !tile_active(active, gvec + g * maxper + v0, nv)) return;
   double2 *becp = smem;            // [nproj][kVT]
   double2 *w3 = becp + nproj * kVT;  // n3
@@ -907,9 +910,10 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   // thread = (plane wave i of the chunk, vector quarter vq): vectors
   // vl = vq + 4k, k < 4, accumulate together so each projector value is
   // loaded once per thread
-  constexpr int kVQ = 256 / kChunk;
+  constexpr int kVQ = 256 / CH;
   constexpr int kPer = kVT / kVQ;
-  const int i = threadIdx.x % kChunk, vq = threadIdx.x / kChunk;
+  static_assert(kVT % kVQ == 0, "vector groups must divide the tile");
+  const int i = threadIdx.x % CH, vq = threadIdx.x / CH;
   const int64_t e = (int64_t)i0 + i;
   if (e >= d.ld) return;
   const double2 *pg = d.proj + (int64_t)g * nproj * d.ld;
@@ -923,6 +927,10 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
     vv[k] = on[k] ? gvec[g * maxper + v0 + vl] : 0;
     if (on[k] && active && !active[vv[k]]) on[k] = false;
   }
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) any |= on[k];
+  if (!any) return;  // no running vector in this thread's group
   if (e >= npw) {  // padding stays zero
 #pragma unroll
     for (int k = 0; k < kPer; ++k)
@@ -1358,9 +1366,13 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   {
     const size_t lds_asm = sizeof(double2) * ((size_t)nproj_ * kVT + n3_);
     KScope ks("pw_assemble", st, 2);
-    k_pw_assemble2<<<dim3(nchunk, ngroup_, plan_tiles_), 256, lds_asm, st>>>(
-        d, in, in_stride, out, out_stride, d_sigma, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
-        plan_tiles_, nchunk, becp_.p, S_.p, locv);
+    // 32 plane waves x 8 vector groups per block: twice the blocks of 64 x 4
+    // (measured 2 % faster per H-apply on the 48^3 batch than 64 or 16)
+    constexpr int CH = 32;
+    k_pw_assemble2<CH><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, plan_tiles_), 256,
+                         lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
+                                        plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
+                                        becp_.p, S_.p, locv);
   }
   SGW_HIP(hipGetLastError());
 }
