@@ -12,9 +12,9 @@ reference's seed-only test, which leaves some shifts above 1e-8).
 
 A step is one batch of all 8192 systems solved from the right-hand sides
 resident in HBM to the solutions resident in HBM.  By default the timed
-steps' batches go through the solver as one queue (slots refilled as
-systems converge, sgw_batch_solver_run_stream); --stream 0 solves each
-batch in lock step.  After the timed steps every (system, shift) true
+steps' batches go through the solver as one queue of 1536 systems, 512 at
+a time (slots refilled as systems converge, sgw_batch_solver_run_stream);
+--stream 0 solves each batch in lock step.  After the timed steps every (system, shift) true
 residual ||b - (A + sigma) x|| / ||b|| of every timed batch is recomputed in
 FP64 on the device; only solves at <= 1e-8 count.
 
@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--shift-test", type=int, default=1,
                     help="1: a system stops when every shift's residual estimate passes the "
                          "threshold; 0: the reference's test on the seed residual alone")
+    ap.add_argument("--slots", type=int, default=512,
+                    help="solver slots of the streamed solve (0: one batch, 256); more slots "
+                         "put more vectors into every H-apply launch")
     ap.add_argument("--stream", type=int, default=1,
                     help="1: the timed steps' batches pass through the solver as one queue "
                          "(sgw_batch_solver_run_stream), slots refilled as systems finish; "
@@ -463,6 +466,14 @@ def main():
     solver = L.sgw_batch_solver_create(op.handle, ctypes.byref(cfg))
     if not solver:
         raise RuntimeError(L.sgw_last_error().decode())
+    slots = args.slots if args.slots > 0 else nsys
+    solver_q = solver
+    if args.stream and slots != nsys:  # a streaming solver with its own slot count
+        cfg_q = _lib.sgw_batch_config(slots, nshift, ld, args.lmax, 10000, args.threshold,
+                                      args.shift_test, 0.0)
+        solver_q = L.sgw_batch_solver_create(op.handle, ctypes.byref(cfg_q))
+        if not solver_q:
+            raise RuntimeError(L.sgw_last_error().decode())
     stream = torch.cuda.current_stream().cuda_stream
     ierr = np.zeros(nsys, dtype=np.int32)
     iters = np.zeros(nsys, dtype=np.int32)
@@ -496,7 +507,7 @@ def main():
 
     def stream_run(nbatch):
         n = nbatch * nsys
-        st = L.sgw_batch_solver_run_stream(solver, n, ctypes.c_void_p(h_group_q.ctypes.data),
+        st = L.sgw_batch_solver_run_stream(solver_q, n, ctypes.c_void_p(h_group_q.ctypes.data),
                                            ctypes.c_void_p(d_rhs_q.data_ptr()),
                                            ctypes.c_void_p(d_sigma_q.data_ptr()),
                                            ctypes.c_void_p(h_sigma_q.ctypes.data),
@@ -607,7 +618,8 @@ def main():
             prof = json.load(f)
         if prof.get("workload") == [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold,
                                     args.shift_test, relax, args.stream,
-                                    args.steps if args.stream else 1]:
+                                    args.steps if args.stream else 1,
+                                    slots if args.stream else nsys]:
             traffic = prof["traffic_bytes_per_happly"]
             traffic_note = {"source": "profiles/happly_traffic.json",
                             "algorithmic_bytes_per_launch": round(bh * total_vec_happly
@@ -640,7 +652,7 @@ def main():
             "max_true_residual": res_max,
             "iterations_max": int(iters.max()), "iterations_median": float(np.median(iters)),
             "execution": (f"streamed: the {args.steps} batches as one queue of {nq} systems "
-                          f"through {nsys} slots" if args.stream else "lock-step per batch"),
+                          f"through {slots} slots" if args.stream else "lock-step per batch"),
             "setup_s": round(setup_s, 2),
         },
         "roofline": {
@@ -659,6 +671,12 @@ def main():
             "avg_launch_ms": round(t_h * 1e3 / max(launches, 1), 4),
             "happly_time_share": round(t_h / elapsed, 4),
             "fp64_tflops_equiv": round(fh * total_vec_happly / t_h / 1e12, 2) if t_h else 0.0,
+            # achieved is B_H, the bytes of a three-pass full-box pipeline; the
+            # sphere-pruned pipeline moves a fraction of them (traffic), so frac
+            # can pass 1.  measured_hbm_GBps is the counted traffic per launch
+            # over the launch time.
+            "measured_hbm_GBps": (round(traffic / (t_h / max(launches, 1)) / 1e9, 1)
+                                  if traffic and t_h else None),
         },
     }
     if breakdown is not None:
@@ -677,6 +695,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w, args.threshold, args.lmax, iters.copy(),
                                            args.shift_test)
     print(json.dumps(out))
+    if solver_q != solver:
+        L.sgw_batch_solver_destroy(solver_q)
     L.sgw_batch_solver_destroy(solver)
 
 
