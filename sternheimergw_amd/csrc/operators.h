@@ -93,6 +93,9 @@ class PwOperator : public Operator {
   // paired planes with the transforms split over lane pairs (SGW_PW_SPLIT=0
   // keeps one lane per row or column)
   bool split_planes_ = true;
+  // split plane transforms specialised to the sphere's half-width when it is
+  // one of the usual ones for the box (SGW_PW_BAND=0 keeps the general kernel)
+  bool band_planes_ = true;
   void ensure_scratch(int nvec);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;

@@ -557,14 +557,21 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
 //   Y' as Y, y -> m2, into the sphere columns S[v][z][c].
 // LDS holds P[2][nc1][N]; the column values T[2][nslot] are staged into its
 // start and read into registers before the Y stage overwrites them.
-template <int N>
+// MB >= 0: the sphere's half-widths M1 = M2 = MB are compile-time constants.
+// Every transform input outside |m| <= MB is then a literal zero and every
+// output outside it is never stored, so the compiler drops the butterflies
+// that only touch those (the file is built without signed zeros and
+// non-finite values, which lets x + 0 and x * 0 fold).  MB = -1: run-time
+// half-widths.
+template <int N, int MB>
 __device__ inline void plane_split_body(const PwDev &d, const double2 *__restrict__ Tin,
                                         const int32_t *__restrict__ group,
                                         const int32_t *__restrict__ active,
                                         double2 *__restrict__ S, int z, int v0, int nvec,
                                         double2 *smem) {
   constexpr int H = N / 2, Q = N / 4;
-  const int nc1 = d.nc1, nc2 = d.nc2, nslot = d.nslot, M2 = d.M2;
+  const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
+  const int nslot = d.nslot, M2 = MB >= 0 ? MB : d.M2;
   const int t = threadIdx.x, r = t >> 1;
   const bool odd = (t & 1) != 0;
   bool live[2];
@@ -606,8 +613,11 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
       if (!live[p]) continue;
       // per-plane copies the compiler cannot see through, so that addresses
       // and masks are formed inside the loop rather than kept live across it
-      int y = r, M1 = d.M1;
-      __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
+      int y = r, M1 = MB >= 0 ? MB : d.M1;
+      if constexpr (MB >= 0)
+        __asm__ volatile("" : "+v"(y)::"memory");
+      else
+        __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
       double2 *Pp = P + p * nc1 * N + y;
 #pragma unroll
       for (int n = 0; n < H; ++n) {
@@ -970,6 +980,13 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   }
 }
 
+// f(std::integral_constant<int, M>) for the M of the list equal to m; false if none
+template <int... Ms, typename F> inline bool dispatch_band(int m, F &&f) {
+  bool hit = false;
+  ((m == Ms ? (f(std::integral_constant<int, Ms>{}), hit = true) : false), ...);
+  return hit;
+}
+
 // Two independent stages of one FP64 application in one launch each, which
 // saves a launch boundary (and the tail of the smaller stage) per pair:
 //   k_pw_becp_zinv:    blocks [0, nbecp) are projector partial sums (256
@@ -1028,8 +1045,9 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
 }
 
 // As k_pw_reduce_plane with the split plane transforms (128-lane blocks):
-// a reduction block covers two of the 64-output ranges.
-template <int N>
+// a reduction block covers two of the 64-output ranges.  MB as in
+// plane_split_body.
+template <int N, int MB>
 __global__ __launch_bounds__(128) void k_pw_reduce_split(
     PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
@@ -1046,7 +1064,7 @@ __global__ __launch_bounds__(128) void k_pw_reduce_split(
     return;
   }
   const int pb = b - nred;
-  plane_split_body<N>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
+  plane_split_body<N, MB>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
 }
 
 }  // namespace
@@ -1112,6 +1130,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
   if (const char *f = std::getenv("SGW_PW_FUSE")) fuse_stages_ = std::strcmp(f, "0") != 0;
   if (const char *f = std::getenv("SGW_PW_PAIR")) pair_planes_ = std::strcmp(f, "0") != 0;
   if (const char *f = std::getenv("SGW_PW_SPLIT")) split_planes_ = std::strcmp(f, "0") != 0;
+  if (const char *f = std::getenv("SGW_PW_BAND")) band_planes_ = std::strcmp(f, "0") != 0;
   SGW_REQUIRE(n1_ > 0 && n2_ > 0 && n3_ > 0 && n1_ <= 512 && n2_ <= 512 && n3_ <= 512, SGW_E_ARG,
               "pw operator: FFT box dimensions must be in [1, 512]");
   ngroup_ = desc.ngroup;
@@ -1287,10 +1306,21 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
           const size_t lds_split = sizeof(double2) * 2 * (size_t)nc1_ * NB;
           if (split_planes_ && pair_planes_ && 2 * nc1_ <= 64 && lds_split <= 64 * 1024) {
             const int64_t nred_s = (int64_t)((nred_x + 1) / 2) * ngroup_ * tiles;
-            k_pw_reduce_split<NB>
-                <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + 1) / 2)), 128, lds_split, st>>>(
-                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
-                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+            auto launch_split = [&](auto mb) {
+              k_pw_reduce_split<NB, decltype(mb)::value>
+                  <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + 1) / 2)), 128, lds_split, st>>>(
+                      d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
+                      ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+            };
+            // compile-time sphere half-widths for the usual ones of this box
+            bool banded = false;
+            if (band_planes_ && M1_ == M2_) {
+              if constexpr (NB == 48) banded = dispatch_band<8, 9, 10, 11>(M1_, launch_split);
+              if constexpr (NB == 36) banded = dispatch_band<7, 8>(M1_, launch_split);
+              if constexpr (NB == 32) banded = dispatch_band<6, 7>(M1_, launch_split);
+              if constexpr (NB == 24) banded = dispatch_band<4, 5>(M1_, launch_split);
+            }
+            if (!banded) launch_split(std::integral_constant<int, -1>{});
             paired = true;
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>

@@ -64,12 +64,14 @@ def test_pw_apply_multigroup(multi_q):
         assert np.linalg.norm(ax - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("env", ["SGW_PW_SPLIT", "SGW_PW_PAIR", "SGW_PW_FUSE"])
+@pytest.mark.parametrize("env", ["SGW_PW_BAND", "SGW_PW_SPLIT", "SGW_PW_PAIR", "SGW_PW_FUSE"])
 def test_pw_apply_launch_layouts_agree(multi_q, monkeypatch, env):
     """The FP64 cubic path's launch layouts: the default (projector stages
     sharing launches with the transforms, two vectors per plane block, the
-    plane transforms split over lane pairs) and, with `env`=0, one lane per
-    row or column, one vector per plane block or separate kernels per stage.
+    plane transforms split over lane pairs and specialised to the sphere's
+    half-width, here 4 on the 24^3 box) and, with `env`=0, the general
+    half-width, one lane per row or column, one vector per plane block or
+    separate kernels per stage.
     An odd number of vectors over three groups leaves a half-filled last
     pair.  Both layouts match the oracle to 1e-12 and each other to 1e-14
     (they may round differently where FMAs are formed differently)."""
