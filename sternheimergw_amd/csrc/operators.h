@@ -103,7 +103,7 @@ class PwOperator : public Operator {
   DevBuf<int32_t> plan_vec_, plan_cnt_;
   DevBuf<double2> part_, becp_;
   int n1_, n2_, n3_, ngroup_, nproj_;
-  int M1_, M2_, nc1_, nc2_, nslot_;
+  int M1_, M2_, M3_ = 0, nc1_, nc2_, nslot_;
   std::vector<int32_t> npw_;
   DevBuf<double> vloc_, vloc_zxy_, kin_, dcoef_;
   DevBuf<int32_t> d_npw_, col_start_, col_cnt_, col_lo_, sidx_, sm3_, slot_of_, m3_of_;

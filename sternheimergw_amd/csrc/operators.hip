@@ -289,8 +289,10 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
 // FP64 z transforms with each column split over a lane pair (splitfft.h):
 // lane 2c + odd of the pair of column c holds N/2 of its values, so these
 // need about half the registers of k_pw_zinv / k_pw_zfwd.  Both lanes of a
-// pair take the same early exits.
-template <int N>
+// pair take the same early exits.  MB >= 0: the sphere's z half-width M3 as
+// a compile-time constant, so values outside |m3| <= MB are literal zeros
+// (inverse) or never stored (forward); -1: no such bound.
+template <int N, int MB>
 __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict__ in,
                                        int64_t sin, const int32_t *__restrict__ group,
                                        const int32_t *__restrict__ active,
@@ -317,7 +319,8 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
 #pragma unroll
   for (int n = 0; n < H; ++n) {
     const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k = m - lo;
-    b[n] = (k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
+    const bool band = MB < 0 || (m >= -MB && m <= MB);
+    b[n] = (band && k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
   }
   split_dit<N, +1>(b, odd);
 #pragma unroll
@@ -327,7 +330,7 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
   }
 }
 
-template <int N>
+template <int N, int MB>
 __global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const double2 *__restrict__ S,
                                                       const int32_t *__restrict__ group,
                                                       const int32_t *__restrict__ active,
@@ -356,7 +359,8 @@ __global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const double2 *__
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k3 = m - lo;
-    if (k3 >= 0 && k3 < cnt) lv[sidx[k3]] = mk(b[k].x * sc, b[k].y * sc);
+    const bool band = MB < 0 || (m >= -MB && m <= MB);
+    if (band && k3 >= 0 && k3 < cnt) lv[sidx[k3]] = mk(b[k].x * sc, b[k].y * sc);
   }
 }
 
@@ -987,6 +991,20 @@ template <int... Ms, typename F> inline bool dispatch_band(int m, F &&f) {
   return hit;
 }
 
+// f(std::integral_constant<int, M>) with M = m when m is one of the sphere
+// half-widths specialised for box N (the usual ones: the box is the smallest
+// FFT size of at least 4M + 1, or one size up), else with M = -1
+template <int N, typename F> inline void with_band(bool enable, int m, F &&f) {
+  bool hit = false;
+  if (enable) {
+    if constexpr (N == 48) hit = dispatch_band<8, 9, 10, 11>(m, f);
+    if constexpr (N == 36) hit = dispatch_band<7, 8>(m, f);
+    if constexpr (N == 32) hit = dispatch_band<6, 7>(m, f);
+    if constexpr (N == 24) hit = dispatch_band<4, 5>(m, f);
+  }
+  if (!hit) f(std::integral_constant<int, -1>{});
+}
+
 // Two independent stages of one FP64 application in one launch each, which
 // saves a launch boundary (and the tail of the smaller stage) per pair:
 //   k_pw_becp_zinv:    blocks [0, nbecp) are projector partial sums (256
@@ -997,7 +1015,7 @@ template <int... Ms, typename F> inline bool dispatch_band(int m, F &&f) {
 // on it is uniform within a block, so the barriers of either part are safe.
 // SPLIT: the z inverse transforms split over lane pairs (32 columns per
 // 64-lane group), else one lane per column.
-template <int N, bool SPLIT>
+template <int N, bool SPLIT, int MB>
 __global__ __launch_bounds__(256) void k_pw_becp_zinv(
     PwDev d, const double2 *__restrict__ in, int64_t sin, const int32_t *__restrict__ group,
     const int32_t *__restrict__ active, const int32_t *__restrict__ gvec,
@@ -1015,8 +1033,8 @@ __global__ __launch_bounds__(256) void k_pw_becp_zinv(
   if (zb >= gx * nvec) return;
   const int lane = (int)(threadIdx.x & 63);
   if constexpr (SPLIT)
-    zinv_split_body<N>(d, in, sin, group, active, T, zb / gx, (zb % gx) * 32 + (lane >> 1),
-                       (lane & 1) != 0);
+    zinv_split_body<N, MB>(d, in, sin, group, active, T, zb / gx, (zb % gx) * 32 + (lane >> 1),
+                           (lane & 1) != 0);
   else
     zinv_body<N, double2>(d, in, sin, group, active, T, nullptr, 0, zb / gx,
                           (zb % gx) * 64 + lane);
@@ -1152,7 +1170,7 @@ PwOperator::PwOperator(const PwDesc &desc) {
   SGW_REQUIRE(2 * M1 + 1 <= n1_ && 2 * M2 + 1 <= n2_ && 2 * M3 + 1 <= n3_, SGW_E_SHAPE,
               "pw operator: the plane-wave sphere does not fit in the FFT box");
   n = npw_[0];
-  M1_ = M1; M2_ = M2;
+  M1_ = M1; M2_ = M2; M3_ = M3;
   nc1_ = 2 * M1 + 1; nc2_ = 2 * M2 + 1;
   nslot_ = nc1_ * nc2_;
   // column tables
@@ -1288,11 +1306,14 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       {
         KScope ks("pw_becp_zinv", st, 2);
         if (zsplit)
-          k_pw_becp_zinv<NB, true><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
-              d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
-              nchunk, ngroup_, part_.p, T_.p, nvec);
+          with_band<NB>(band_planes_, M3_, [&](auto mb) {
+            k_pw_becp_zinv<NB, true, decltype(mb)::value>
+                <<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+                    d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
+                    tiles, nchunk, ngroup_, part_.p, T_.p, nvec);
+          });
         else
-          k_pw_becp_zinv<NB, false><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+          k_pw_becp_zinv<NB, false, -1><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
               d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
               nchunk, ngroup_, part_.p, T_.p, nvec);
       }
@@ -1313,14 +1334,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
                       ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
             };
             // compile-time sphere half-widths for the usual ones of this box
-            bool banded = false;
-            if (band_planes_ && M1_ == M2_) {
-              if constexpr (NB == 48) banded = dispatch_band<8, 9, 10, 11>(M1_, launch_split);
-              if constexpr (NB == 36) banded = dispatch_band<7, 8>(M1_, launch_split);
-              if constexpr (NB == 32) banded = dispatch_band<6, 7>(M1_, launch_split);
-              if constexpr (NB == 24) banded = dispatch_band<4, 5>(M1_, launch_split);
-            }
-            if (!banded) launch_split(std::integral_constant<int, -1>{});
+            with_band<NB>(band_planes_ && M1_ == M2_, M1_, launch_split);
             paired = true;
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>
@@ -1339,8 +1353,10 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       {
         KScope ks("pw_zfwd", st, 2);
         if (zsplit)
-          k_pw_zfwd_split<NB><<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, S_.p, d_group,
-                                                                         d_active, loc_.p);
+          with_band<NB>(band_planes_, M3_, [&](auto mb) {
+            k_pw_zfwd_split<NB, decltype(mb)::value>
+                <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p);
+          });
         else
           k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, nullptr, 0);
       }
