@@ -551,46 +551,50 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
   }
 }
 
-// FP64 plane transforms of two vectors, planes (z, v0) and (z, v0 + 1), by a
-// block of 128 lanes whose transforms are split over lane pairs (splitfft.h):
-// every lane holds N/2 values, so the block needs about half the registers
-// per lane of plane2_body and the SIMDs hold three wavefronts instead of one.
-//   Y  lane pair r < 2 nc1: row a of plane yp (r = yp nc1 + a), m2 -> y;
-//   X  lane pair r < N: column y = r of each live plane in turn, m1 -> x,
-//      times V(x, y, z), x -> m1 (the nc1 sphere frequencies kept);
+// FP64 plane transforms of PL vectors, planes (z, v0 + p) for p < PL, by a
+// block of 64 W lanes whose transforms are split over lane pairs
+// (splitfft.h): every lane holds N/2 values, so the block needs about half
+// the registers per lane of plane2_body and each SIMD holds two wavefronts.
+//   Y  lane pair r < PL nc1: row a of plane yp (r = yp nc1 + a), m2 -> y;
+//   X  the PL N (plane, column) tasks in rounds of 32 W lane pairs: column y
+//      of plane p, m1 -> x, times V(x, y, z), x -> m1 (nc1 frequencies kept);
 //   Y' as Y, y -> m2, into the sphere columns S[v][z][c].
-// LDS holds P[2][nc1][N]; the column values T[2][nslot] are staged into its
-// start and read into registers before the Y stage overwrites them.
+// With PL N a multiple of 32 W (4 planes of 48 on 3 wavefronts) every lane
+// pair works in every X round.  LDS holds P[PL][nc1][N]; the column values
+// T[PL][nslot] are staged into its start and read into registers before the
+// Y stage overwrites them.
 // MB >= 0: the sphere's half-widths M1 = M2 = MB are compile-time constants.
 // Every transform input outside |m| <= MB is then a literal zero and every
 // output outside it is never stored, so the compiler drops the butterflies
 // that only touch those (the file is built without signed zeros and
 // non-finite values, which lets x + 0 and x * 0 fold).  MB = -1: run-time
 // half-widths.
-template <int N, int MB>
+template <int N, int MB, int PL, int W>
 __device__ inline void plane_split_body(const PwDev &d, const double2 *__restrict__ Tin,
                                         const int32_t *__restrict__ group,
                                         const int32_t *__restrict__ active,
                                         double2 *__restrict__ S, int z, int v0, int nvec,
                                         double2 *smem) {
-  constexpr int H = N / 2, Q = N / 4;
+  constexpr int H = N / 2, Q = N / 4, NP = 32 * W;  // NP lane pairs
   const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
   const int nslot = d.nslot, M2 = MB >= 0 ? MB : d.M2;
   const int t = threadIdx.x, r = t >> 1;
   const bool odd = (t & 1) != 0;
-  bool live[2];
+  bool live[PL], any = false;
 #pragma unroll
-  for (int p = 0; p < 2; ++p) live[p] = v0 + p < nvec && (!active || active[v0 + p]);
-  if (!live[0] && !live[1]) return;
-  double2 *P = smem, *T = smem;  // T[2][nslot] aliases the start of P[2][nc1][N]
-  if (t < 64) {  // one wavefront stages both planes' columns
-#pragma unroll
-    for (int p = 0; p < 2; ++p)
-      if (live[p]) stage_plane(Tin + ((int64_t)(v0 + p) * N + z) * nslot, T + p * nslot, nslot);
+  for (int p = 0; p < PL; ++p) {
+    live[p] = v0 + p < nvec && (!active || active[v0 + p]);
+    any |= live[p];
   }
+  if (!any) return;
+  double2 *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][N]
+#pragma unroll
+  for (int p = 0; p < PL; ++p)  // plane p staged by wavefront p mod W
+    if ((t >> 6) == p % W && live[p])
+      stage_plane(Tin + ((int64_t)(v0 + p) * N + z) * nslot, T + p * nslot, nslot);
   __syncthreads();
-  const int yp = r < nc1 ? 0 : 1, a = r - yp * nc1;
-  const bool yrow = a < nc1 && live[yp];
+  const int yp = r / nc1, a = r - yp * nc1;
+  const bool yrow = yp < PL && live[yp];
   double2 b[H];
   if (yrow) {
     const double2 *Tp = T + yp * nslot + a * nc2 + M2;
@@ -611,37 +615,37 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
     }
   }
   __syncthreads();
-  if (r < N) {  // X inverse, V(r), X forward of column y = r, plane by plane
+  // X inverse, V(r), X forward: task = (plane p, column y), round by round
 #pragma unroll 1
-    for (int p = 0; p < 2; ++p) {
-      if (!live[p]) continue;
-      // per-plane copies the compiler cannot see through, so that addresses
-      // and masks are formed inside the loop rather than kept live across it
-      int y = r, M1 = MB >= 0 ? MB : d.M1;
-      if constexpr (MB >= 0)
-        __asm__ volatile("" : "+v"(y)::"memory");
-      else
-        __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
-      double2 *Pp = P + p * nc1 * N + y;
+  for (int task = r; task < PL * N; task += NP) {
+    const int p = task / N;
+    if (!live[p]) continue;
+    // per-round copies the compiler cannot see through, so that addresses
+    // and masks are formed inside the loop rather than kept live across it
+    int y = task - p * N, M1 = MB >= 0 ? MB : d.M1;
+    if constexpr (MB >= 0)
+      __asm__ volatile("" : "+v"(y)::"memory");
+    else
+      __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
+    double2 *Pp = P + p * nc1 * N + y;
 #pragma unroll
-      for (int n = 0; n < H; ++n) {
-        const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-        b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N] : mk(0, 0);
-      }
-      split_dit_sel<N, +1>(b, odd);
-      const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
+    for (int n = 0; n < H; ++n) {
+      const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+      b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N] : mk(0, 0);
+    }
+    split_dit_sel<N, +1>(b, odd);
+    const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
 #pragma unroll
-      for (int j = 0; j < Q; ++j) {  // x = j (+Q) and H + j (+Q)
-        const double v1 = vcol[j * N], v2 = vcol[(H + j) * N];
-        b[j] = mk(b[j].x * v1, b[j].y * v1);
-        b[Q + j] = mk(b[Q + j].x * v2, b[Q + j].y * v2);
-      }
-      split_dif_sel<N, -1>(b, odd);
+    for (int j = 0; j < Q; ++j) {  // x = j (+Q) and H + j (+Q)
+      const double v1 = vcol[j * N], v2 = vcol[(H + j) * N];
+      b[j] = mk(b[j].x * v1, b[j].y * v1);
+      b[Q + j] = mk(b[Q + j].x * v2, b[Q + j].y * v2);
+    }
+    split_dif_sel<N, -1>(b, odd);
 #pragma unroll
-      for (int k = 0; k < H; ++k) {
-        const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-        if (m >= -M1 && m <= M1) Pp[(m + M1) * N] = b[k];
-      }
+    for (int k = 0; k < H; ++k) {
+      const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+      if (m >= -M1 && m <= M1) Pp[(m + M1) * N] = b[k];
     }
   }
   __syncthreads();
@@ -1062,28 +1066,37 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
     plane_body<N, double2>(d, T, group, active, S, nullptr, 0, pb % N, pb / N, smem_d);
 }
 
-// As k_pw_reduce_plane with the split plane transforms (128-lane blocks):
-// a reduction block covers two of the 64-output ranges.  MB as in
-// plane_split_body.
-template <int N, int MB>
-__global__ __launch_bounds__(128) void k_pw_reduce_split(
+// As k_pw_reduce_plane with the split plane transforms (blocks of W
+// wavefronts, PL vectors per plane block): a reduction block covers W of the
+// 64-output ranges.  MB as in plane_split_body.
+template <int N, int MB, int PL, int W>
+__global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
     PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
     int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
     double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S,
     int nvec) {
   extern __shared__ double2 smem_d[];
-  const int nred_h = (nred_x + 1) / 2, nred = nred_h * ngroup * tiles, b = blockIdx.x;
+  const int nred_w = (nred_x + W - 1) / W, nred = nred_w * ngroup * tiles, b = blockIdx.x;
   if (b < nred) {
-    const int bx = 2 * (b % nred_h) + (int)(threadIdx.x >> 6);
+    const int bx = W * (b % nred_w) + (int)(threadIdx.x >> 6);
     if (bx < nred_x)
       becp_reduce_body(d, active, gvec, gcnt, maxper, tiles, nchunk, part, becp, bx,
-                       (b / nred_h) % ngroup, b / (nred_h * ngroup), threadIdx.x & 63);
+                       (b / nred_w) % ngroup, b / (nred_w * ngroup), threadIdx.x & 63);
     return;
   }
   const int pb = b - nred;
-  plane_split_body<N, MB>(d, T, group, active, S, pb % N, 2 * (pb / N), nvec, smem_d);
+  plane_split_body<N, MB, PL, W>(d, T, group, active, S, pb % N, PL * (pb / N), nvec, smem_d);
 }
+
+// planes per block and wavefronts per block of the split plane kernel.  Four
+// planes on three wavefronts keep every lane pair busy in the 48^3 X stage
+// but hold 52 KB of LDS per block; measured on the 48^3 batch, two planes on
+// two wavefronts are 4 % faster per H-apply.
+template <int N> struct SplitShape {
+  static constexpr int PL = 2;
+  static constexpr int W = 2;
+};
 
 }  // namespace
 
@@ -1323,18 +1336,24 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
         // and four such blocks still fit in a CU's LDS (N = 64 would spill)
         bool paired = false;
         if constexpr (NB <= 48) {
-          // split transforms over lane pairs (LDS: P of both planes only)
-          const size_t lds_split = sizeof(double2) * 2 * (size_t)nc1_ * NB;
-          if (split_planes_ && pair_planes_ && 2 * nc1_ <= 64 && lds_split <= 64 * 1024) {
-            const int64_t nred_s = (int64_t)((nred_x + 1) / 2) * ngroup_ * tiles;
-            auto launch_split = [&](auto mb) {
-              k_pw_reduce_split<NB, decltype(mb)::value>
-                  <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + 1) / 2)), 128, lds_split, st>>>(
-                      d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
-                      ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
-            };
+          // split transforms over lane pairs (LDS: P of the block's planes only)
+          constexpr int PL = SplitShape<NB>::PL, W = SplitShape<NB>::W;
+          const size_t lds_split = sizeof(double2) * PL * (size_t)nc1_ * NB;
+          auto launch_split = [&](auto shape, auto mb) {
+            constexpr int pl = decltype(shape)::value / 8, w = decltype(shape)::value % 8;
+            const int64_t nred_s = (int64_t)((nred_x + w - 1) / w) * ngroup_ * tiles;
+            k_pw_reduce_split<NB, decltype(mb)::value, pl, w>
+                <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + pl - 1) / pl)), 64 * w,
+                   sizeof(double2) * pl * (size_t)nc1_ * NB, st>>>(
+                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
+                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+          };
+          const bool band = band_planes_ && M1_ == M2_;
+          if (split_planes_ && pair_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
             // compile-time sphere half-widths for the usual ones of this box
-            with_band<NB>(band_planes_ && M1_ == M2_, M1_, launch_split);
+            with_band<NB>(band, M1_, [&](auto mb) {
+              launch_split(std::integral_constant<int, PL * 8 + W>{}, mb);
+            });
             paired = true;
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>
