@@ -12,8 +12,10 @@ reference's seed-only test, which leaves some shifts above 1e-8).
 
 A step is one batch of all 8192 systems solved from the right-hand sides
 resident in HBM to the solutions resident in HBM.  By default the timed
-steps' batches go through the solver as one queue of 1536 systems, 512 at
-a time (slots refilled as systems converge, sgw_batch_solver_run_stream);
+steps' batches go through the solver as one queue (20 steps: 5120 systems),
+512 at a time (slots refilled as systems converge,
+sgw_batch_solver_run_stream); the end of the queue, where the slow systems
+finish with few others beside them, weighs less the longer the queue;
 --stream 0 solves each batch in lock step.  After the timed steps every (system, shift) true
 residual ||b - (A + sigma) x|| / ||b|| of every timed batch is recomputed in
 FP64 on the device; only solves at <= 1e-8 count.
@@ -41,7 +43,7 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "happly_traffic.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--threshold", type=float, default=1e-8)
     ap.add_argument("--lmax", type=int, default=4)
