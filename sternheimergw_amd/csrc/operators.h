@@ -87,12 +87,12 @@ class PwOperator : public Operator {
   // FP64 cubic path: the projector stages share launches with the z inverse
   // and plane transforms (SGW_PW_FUSE=0 in the environment keeps them apart)
   bool fuse_stages_ = true;
-  // fused path: two vectors per plane block where they fit (SGW_PW_PAIR=0
-  // keeps one)
-  bool pair_planes_ = true;
-  // paired planes with the transforms split over lane pairs (SGW_PW_SPLIT=0
-  // keeps one lane per row or column)
+  // fused path: the plane and z transforms split over lane pairs
+  // (SGW_PW_SPLIT=0 keeps one lane per row or column)
   bool split_planes_ = true;
+  // unsplit plane transforms: two vectors per plane block where they fit
+  // (SGW_PW_PAIR=0 keeps one)
+  bool pair_planes_ = true;
   // split plane transforms specialised to the sphere's half-width when it is
   // one of the usual ones for the box (SGW_PW_BAND=0 keeps the general kernel)
   bool band_planes_ = true;

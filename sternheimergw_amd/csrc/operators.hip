@@ -1089,13 +1089,14 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
   plane_split_body<N, MB, PL, W>(d, T, group, active, S, pb % N, PL * (pb / N), nvec, smem_d);
 }
 
-// planes per block and wavefronts per block of the split plane kernel.  Four
-// planes on three wavefronts keep every lane pair busy in the 48^3 X stage
-// but hold 52 KB of LDS per block; measured on the 48^3 batch, two planes on
-// two wavefronts are 4 % faster per H-apply.
+// planes per block and wavefronts per block of the split plane kernel.
+// Measured per H-apply on the 48^3 batch: one plane on one wavefront (the X
+// stage in 1.5 rounds of 32 lane pairs, 13 KB of LDS) is 4 % faster than two
+// planes on two wavefronts, 12 % faster than two on three and 15 % faster
+// than four on three (every lane pair busy, but 52 KB of LDS per block).
 template <int N> struct SplitShape {
-  static constexpr int PL = 2;
-  static constexpr int W = 2;
+  static constexpr int PL = 1;
+  static constexpr int W = 1;
 };
 
 }  // namespace
@@ -1332,8 +1333,10 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
       }
       {
         KScope ks("pw_reduce_plane", st, 2);
-        // two vectors per plane block while both Y stages fit in one wavefront
-        // and four such blocks still fit in a CU's LDS (N = 64 would spill)
+        // the transforms split over lane pairs (N <= 48); else, without the
+        // split, two vectors per plane block while both Y stages fit in one
+        // wavefront and four such blocks still fit in a CU's LDS (N = 64
+        // would spill)
         bool paired = false;
         if constexpr (NB <= 48) {
           // split transforms over lane pairs (LDS: P of the block's planes only)
@@ -1349,7 +1352,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
                     ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
           };
           const bool band = band_planes_ && M1_ == M2_;
-          if (split_planes_ && pair_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
+          if (split_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
             // compile-time sphere half-widths for the usual ones of this box
             with_band<NB>(band, M1_, [&](auto mb) {
               launch_split(std::integral_constant<int, PL * 8 + W>{}, mb);

@@ -64,14 +64,15 @@ def test_pw_apply_multigroup(multi_q):
         assert np.linalg.norm(ax - ref) <= 1e-12 * np.linalg.norm(ref)
 
 
-@pytest.mark.parametrize("env", ["SGW_PW_BAND", "SGW_PW_SPLIT", "SGW_PW_PAIR", "SGW_PW_FUSE"])
+@pytest.mark.parametrize("env", [{"SGW_PW_BAND": "0"}, {"SGW_PW_SPLIT": "0"},
+                                 {"SGW_PW_SPLIT": "0", "SGW_PW_PAIR": "0"},
+                                 {"SGW_PW_FUSE": "0"}])
 def test_pw_apply_launch_layouts_agree(multi_q, monkeypatch, env):
-    """The FP64 cubic path's launch layouts: the default (projector stages
-    sharing launches with the transforms, two vectors per plane block, the
-    plane transforms split over lane pairs and specialised to the sphere's
-    half-width, here 4 on the 24^3 box) and, with `env`=0, the general
-    half-width, one lane per row or column, one vector per plane block or
-    separate kernels per stage.
+    """The FP64 cubic path's launch layouts.  The default: projector stages
+    sharing launches with the transforms, transforms split over lane pairs
+    and specialised to the sphere's half-width (here 4 on the 24^3 box).
+    With `env`: the general half-width; one lane per row or column (two
+    vectors per plane block, or one); separate kernels per stage.
     An odd number of vectors over three groups leaves a half-filled last
     pair.  Both layouts match the oracle to 1e-12 and each other to 1e-14
     (they may round differently where FMAs are formed differently)."""
@@ -84,16 +85,17 @@ def test_pw_apply_launch_layouts_agree(multi_q, monkeypatch, env):
         x[v, multi_q.npw[grp[v]]:] = 0
     sig = 0.1 - 0.7j
     out = {}
-    for val in ("1", "0"):
-        monkeypatch.setenv(env, val)
-        out[val] = operators.PwOperator.from_workload(multi_q).apply(x, sig, group=grp)
+    out["default"] = operators.PwOperator.from_workload(multi_q).apply(x, sig, group=grp)
+    for k, val in env.items():
+        monkeypatch.setenv(k, val)
+    out["variant"] = operators.PwOperator.from_workload(multi_q).apply(x, sig, group=grp)
     for v in range(nv):
         n = multi_q.npw[grp[v]]
         ref = orc.PwOp(multi_q, int(grp[v])).apply(x[v, :n], sig)
         scale = np.linalg.norm(ref)
-        for val in ("1", "0"):
-            assert np.linalg.norm(out[val][v, :n] - ref) <= 1e-12 * scale, (val, v)
-        assert np.linalg.norm(out["1"][v] - out["0"][v]) <= 1e-14 * scale, v
+        for key in out:
+            assert np.linalg.norm(out[key][v, :n] - ref) <= 1e-12 * scale, (key, v)
+        assert np.linalg.norm(out["default"][v] - out["variant"][v]) <= 1e-14 * scale, v
 
 
 def test_pw_operator_hermitian(c1):
