@@ -783,6 +783,7 @@ __global__ __launch_bounds__(64) void k_pw_plane_grad(PwDev d, const double2 *__
 // ---------------------------------------------------------------------------
 constexpr int kChunk = 64;
 constexpr int kVT = 16;
+constexpr int kAsmChunk = 32, kAsmThreads = 256;  // k_pw_assemble2 block shape
 constexpr int kPT = 16;  // projectors staged in LDS at a time
 
 // does any vector of the tile still run?  (uniform across the block)
@@ -903,10 +904,10 @@ __device__ inline void becp_body(const PwDev &d, const double2 *__restrict__ in,
 // out_v = kin x_v + fwfft(V invfft x_v) + proj (D becp_v) + sigma_v x_v for the
 // plane waves of one chunk and the vectors of one tile; becp_v is the sum of
 // the chunk partials, the local term the z forward transform of S.
-// CH plane waves per block: thread = (plane wave i < CH, vector group vq <
-// 256 / CH), each thread kVT / (256 / CH) vectors.
-template <int CH>
-__global__ __launch_bounds__(256) void k_pw_assemble2(
+// CH plane waves per block of NT lanes: thread = (plane wave i < CH, vector
+// group vq < NT / CH), each thread kVT / (NT / CH) vectors.
+template <int CH, int NT>
+__global__ __launch_bounds__(NT) void k_pw_assemble2(
     PwDev d, const double2 *__restrict__ in, int64_t sin, double2 *__restrict__ out, int64_t sout,
     const double2 *__restrict__ sigma, const int32_t *__restrict__ active,
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
@@ -922,13 +923,13 @@ __global__ __launch_bounds__(256) void k_pw_assemble2(
   double2 *becp = smem;            // [nproj][kVT]
   double2 *w3 = becp + nproj * kVT;  // n3
   const double2 *bsrc = part + ((int64_t)g * tiles + vt) * nproj * kVT;  // reduced D becp
-  for (int o = threadIdx.x; o < nproj * kVT; o += 256) becp[o] = bsrc[o];
-  for (int i = threadIdx.x; i < n3; i += 256) w3[i] = d.tw3[i];
+  for (int o = threadIdx.x; o < nproj * kVT; o += NT) becp[o] = bsrc[o];
+  for (int i = threadIdx.x; i < n3; i += NT) w3[i] = d.tw3[i];
   __syncthreads();
   // thread = (plane wave i of the chunk, vector quarter vq): vectors
   // vl = vq + 4k, k < 4, accumulate together so each projector value is
   // loaded once per thread
-  constexpr int kVQ = 256 / CH;
+  constexpr int kVQ = NT / CH;
   constexpr int kPer = kVT / kVQ;
   static_assert(kVT % kVQ == 0, "vector groups must divide the tile");
   const int i = threadIdx.x % CH, vq = threadIdx.x / CH;
@@ -1434,11 +1435,12 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   {
     const size_t lds_asm = sizeof(double2) * ((size_t)nproj_ * kVT + n3_);
     KScope ks("pw_assemble", st, 2);
-    // 32 plane waves x 8 vector groups per block: twice the blocks of 64 x 4
-    // (measured 2 % faster per H-apply on the 48^3 batch than 64 or 16)
-    constexpr int CH = 32;
-    k_pw_assemble2<CH><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, plan_tiles_), 256,
-                         lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
+    // 32 plane waves x 8 vector groups per 256-lane block: measured per
+    // H-apply on the 48^3 batch, 2 % faster than 64 or 16 plane waves per
+    // 256 lanes and 2-5 % faster than 8 or 16 per 64 lanes
+    constexpr int CH = kAsmChunk, NT = kAsmThreads;
+    k_pw_assemble2<CH, NT><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, plan_tiles_), NT,
+                             lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
                                         plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
                                         becp_.p, S_.p, locv);
   }
