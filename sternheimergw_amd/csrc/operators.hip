@@ -737,6 +737,92 @@ __global__ __launch_bounds__(64) void k_pw_plane_drho(PwDev d, const double2 *__
   }
 }
 
+// The same density response with the transforms split over lane pairs
+// (splitfft.h), one block of 128 lanes per (z, k): lane pair r < N owns
+// column y = r of the plane for all bands, so its 2 x N/2 values of D stay in
+// registers instead of a read-modify-write of D per band.  Per band: the
+// column values are staged in LDS, the Y stage (pairs r < nc1) writes the
+// rows P[nc1][N] over them, the X stage (pairs r < N) transforms column y
+// and accumulates conj(psi_v) dpsi.  MB as in plane_split_body.
+template <int N, int MB>
+__global__ __launch_bounds__(128, 2) void k_pw_plane_drho_split(PwDev d,
+                                                             const double2 *__restrict__ T,
+                                                             int nbnd, int nshift,
+                                                             const double2 *__restrict__ Pr,
+                                                             double wgt, double2 *__restrict__ D,
+                                                             int accumulate) {
+  constexpr int H = N / 2, Q = N / 4;
+  extern __shared__ double2 smem[];  // P[nc1][N]; the column values staged at its start
+  const int z = blockIdx.x, k = blockIdx.y;
+  const int t = threadIdx.x, r = t >> 1;
+  const bool odd = (t & 1) != 0;
+  const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
+  const int nslot = d.nslot, M1 = MB >= 0 ? MB : d.M1, M2 = MB >= 0 ? MB : d.M2;
+  double2 *P = smem, *Ts = smem;
+  const int xo = odd ? Q : 0;  // this lane's x: xo + j and H + xo + j
+  double2 *Dz = D + ((int64_t)k * N + z) * N * N + r;
+  double2 acc[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) acc[j] = mk(0, 0);
+  if (accumulate && r < N) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      acc[j] = Dz[(int64_t)(xo + j) * N];
+      acc[Q + j] = Dz[(int64_t)(H + xo + j) * N];
+    }
+  }
+  for (int v = 0; v < nbnd; ++v) {
+    const int64_t vec = (int64_t)v * nshift + k;
+    __syncthreads();  // the previous band's X stage is done with P
+    if (t < 64) stage_plane(T + (vec * N + z) * nslot, Ts, nslot);
+    __syncthreads();
+    double2 b[H];
+    if (r < nc1) {
+      const double2 *Tr = Ts + r * nc2 + M2;
+#pragma unroll
+      for (int n = 0; n < H; ++n) {
+        const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+        b[n] = (m >= -M2 && m <= M2) ? Tr[m] : mk(0, 0);
+      }
+    }
+    __syncthreads();  // every column value is in registers before P overwrites them
+    if (r < nc1) {  // Y inverse of row r
+      split_dit_sel<N, +1>(b, odd);
+      double2 *Prow = P + r * N + xo;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) {
+        Prow[j] = b[j];
+        Prow[H + j] = b[Q + j];
+      }
+    }
+    __syncthreads();
+    if (r < N) {  // X inverse of column y = r, times conj(psi_v(r)), into D
+      const double2 *Pc = P + r;
+#pragma unroll
+      for (int n = 0; n < H; ++n) {
+        const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
+        b[n] = (m >= -M1 && m <= M1) ? Pc[(m + M1) * N] : mk(0, 0);
+      }
+      split_dit_sel<N, +1>(b, odd);
+      const double2 *pz = Pr + ((int64_t)v * N + z) * N * N + r;
+#pragma unroll
+      for (int j = 0; j < H; ++j) {
+        const int x = (j < Q ? j : H + j - Q) + xo;
+        const double2 p = pz[(int64_t)x * N];
+        acc[j].x = fma(wgt, p.x * b[j].x + p.y * b[j].y, acc[j].x);
+        acc[j].y = fma(wgt, p.x * b[j].y - p.y * b[j].x, acc[j].y);
+      }
+    }
+  }
+  if (r < N) {
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      Dz[(int64_t)(xo + j) * N] = acc[j];
+      Dz[(int64_t)(H + xo + j) * N] = acc[Q + j];
+    }
+  }
+}
+
 // Adjoint gradient of one z-plane over a strided set of vector pairs:
 //   G[b][z][x][y] += sum_{p = b, b + nb, ...} w_p Re(conj(y_p(r)) x_p(r)),
 // from the z-transformed columns Tx (x_p) and Ty (y_p).  Block (z, b) owns
@@ -1477,8 +1563,17 @@ void PwOperator::drho_accumulate(hipStream_t st, int group, int nbnd, int nshift
   auto run = [&](auto tag) {
     constexpr int NB = decltype(tag)::value;
     k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, dpsi, stride, dg.p, nullptr, T_.p);
-    k_pw_plane_drho<NB><<<dim3(NB, nshift), 64, lds, st>>>(d, T_.p, nbnd, nshift, Pr, wgt, D,
-                                                           accumulate ? 1 : 0);
+    if (split_planes_ && nc1_ <= 64) {
+      // transforms split over lane pairs, D of a column in registers
+      with_band<NB>(band_planes_ && M1_ == M2_, M1_, [&](auto mb) {
+        k_pw_plane_drho_split<NB, decltype(mb)::value>
+            <<<dim3(NB, nshift), 128, sizeof(double2) * (size_t)nc1_ * NB, st>>>(
+                d, T_.p, nbnd, nshift, Pr, wgt, D, accumulate ? 1 : 0);
+      });
+    } else {
+      k_pw_plane_drho<NB><<<dim3(NB, nshift), 64, lds, st>>>(d, T_.p, nbnd, nshift, Pr, wgt, D,
+                                                             accumulate ? 1 : 0);
+    }
   };
   switch (n1_) {
     case 24: run(std::integral_constant<int, 24>{}); break;
