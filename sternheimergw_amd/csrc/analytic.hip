@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "boxfft.h"
 #include "fft6.h"
 #include "ktimer.h"
 #include "sgw_internal.h"
@@ -122,12 +123,6 @@ __global__ void k_fill0(int64_t count, double2 *__restrict__ d) {
   if (e < count) d[e] = mk(0.0, 0.0);
 }
 
-__global__ void k_mul_inplace(int64_t count, const double2 *__restrict__ g,
-                              double2 *__restrict__ a) {
-  const int64_t e = (int64_t)blockIdx.x * kEB + threadIdx.x;
-  if (e < count) a[e] = cmul(g[e], a[e]);
-}
-
 // array(r, c) *= alpha for r < num_g, c < num_g_p; sigma(r, c) += array(r, c)
 __global__ void k_alpha_block(int num_g, int num_g_p, int64_t nr, double2 alpha,
                               double2 *__restrict__ a, double2 *__restrict__ sigma,
@@ -237,10 +232,13 @@ extern "C" int sgw_sigma_prod(int32_t n, int32_t n_p, int32_t num_g, const int32
     const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
     DevBuf<double2> S((size_t)num_g * nr_p);
     KScope ks("sigma_prod", st);
-    invfft6_run(st, a, S.p);                                            // sigma.f90:487
-    k_mul_inplace<<<blocks(nr * nr_p), kEB, 0, st>>>(
-        nr * nr_p, reinterpret_cast<const double2 *>(d_green), a.f);     // :499
-    fwfft6_run(st, a, S.p);                                             // :510
+    // invfft6 (sigma.f90:487), the product with G (:499) and fwfft6 (:510);
+    // the r transforms' z passes of both and the product are one kernel
+    invfft6_front(st, a, S.p, 0);
+    box_fft_xy(st, n, nr_p, a.f, -1);
+    box_z_product(st, n, nr_p, a.f, reinterpret_cast<const double2 *>(d_green));
+    box_fft_xy(st, n, nr_p, a.f, +1);
+    fwfft6_back(st, a, S.p, 0);
     k_alpha_block<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
         num_g, num_g_p, nr, make_double2(alpha.re, alpha.im), a.f,
         reinterpret_cast<double2 *>(d_sigma), ld_sigma);                  // :519-521, :741

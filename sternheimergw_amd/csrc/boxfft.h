@@ -13,5 +13,11 @@ const char *box_sizes_text();
 // SIGN = +1: sum e^{+iGr} (QE invfft, no factor); SIGN = -1: sum e^{-iGr}
 // without the 1/n^3 of QE's fwfft.  Throws for unsupported n.
 void box_fft(hipStream_t st, int n, int64_t nvec, double2 *d, int sign);
+// The xy pass of box_fft alone (box_fft = this, then the z pass).
+void box_fft_xy(hipStream_t st, int n, int64_t nvec, double2 *d, int sign);
+// d <- Z_{+1}(g * Z_{-1}(d)) along z for every line: between box_fft_xy(-1)
+// and box_fft_xy(+1), the product of the inverse transform with g followed
+// by the forward transform.
+void box_z_product(hipStream_t st, int n, int64_t nvec, double2 *d, const double2 *g);
 
 }  // namespace sgw

@@ -8,6 +8,8 @@
 #include "boxfft.h"
 #include "regfft.h"
 
+#include <type_traits>
+
 namespace sgw {
 namespace {
 
@@ -54,25 +56,57 @@ __global__ __launch_bounds__(64) void k_box_z(int64_t nvec, double2 *__restrict_
   for (int i = 0; i < N; ++i) p[(int64_t)i * N * N] = x[i];
 }
 
+// The middle of a real-space product between two box transforms: for every
+// z line, d <- Z_{+1}(g * Z_{-1}(d)).  With the xy passes around it,
+// xy(-1), this, xy(+1) is the full inverse transform, the pointwise product
+// with g and the full forward transform: the 3-D DFT is a product of
+// commuting passes along the axes.  g has the layout of d.
+template <int N>
+__global__ __launch_bounds__(64) void k_box_z_mul_z(int64_t nvec, double2 *__restrict__ d,
+                                                    const double2 *__restrict__ g) {
+  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (t >= nvec * N * N) return;
+  const int64_t v = t / (N * N), c = t % (N * N);
+  double2 *p = d + v * N * N * N + c;
+  const double2 *q = g + v * N * N * N + c;
+  double2 x[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) x[i] = p[(int64_t)i * N * N];
+  reg_fft<N, -1>(x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double2 w = q[(int64_t)i * N * N];
+    x[i] = make_double2(x[i].x * w.x - x[i].y * w.y, x[i].x * w.y + x[i].y * w.x);
+  }
+  reg_fft<N, +1>(x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) p[(int64_t)i * N * N] = x[i];
+}
+
 template <int N, int SIGN> void run(hipStream_t st, int64_t nvec, double2 *d) {
   if (nvec <= 0) return;
   k_box_xy<N, SIGN><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
   k_box_z<N, SIGN><<<(unsigned)((nvec * N * N + 63) / 64), 64, 0, st>>>(nvec, d);
 }
 
-template <int SIGN> void dispatch(hipStream_t st, int n, int64_t nvec, double2 *d) {
+// f(std::integral_constant<int, n>) for a supported box side n
+template <typename F> void with_side(int n, F &&f) {
   switch (n) {
-    case 16: run<16, SIGN>(st, nvec, d); break;
-    case 18: run<18, SIGN>(st, nvec, d); break;
-    case 24: run<24, SIGN>(st, nvec, d); break;
-    case 27: run<27, SIGN>(st, nvec, d); break;
-    case 32: run<32, SIGN>(st, nvec, d); break;
-    case 36: run<36, SIGN>(st, nvec, d); break;
-    case 48: run<48, SIGN>(st, nvec, d); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 18: f(std::integral_constant<int, 18>{}); break;
+    case 24: f(std::integral_constant<int, 24>{}); break;
+    case 27: f(std::integral_constant<int, 27>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 36: f(std::integral_constant<int, 36>{}); break;
+    case 48: f(std::integral_constant<int, 48>{}); break;
     default:
       throw Error{SGW_E_SHAPE, std::string("box FFT: cubic boxes of ") + box_sizes_text() +
                                    " points only"};
   }
+}
+
+template <int SIGN> void dispatch(hipStream_t st, int n, int64_t nvec, double2 *d) {
+  with_side(n, [&](auto side) { run<decltype(side)::value, SIGN>(st, nvec, d); });
 }
 
 }  // namespace
@@ -88,6 +122,25 @@ void box_fft(hipStream_t st, int n, int64_t nvec, double2 *d, int sign) {
     dispatch<1>(st, n, nvec, d);
   else
     dispatch<-1>(st, n, nvec, d);
+}
+
+void box_fft_xy(hipStream_t st, int n, int64_t nvec, double2 *d, int sign) {
+  if (nvec <= 0) return;
+  with_side(n, [&](auto side) {
+    constexpr int N = decltype(side)::value;
+    if (sign > 0)
+      k_box_xy<N, 1><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
+    else
+      k_box_xy<N, -1><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
+  });
+}
+
+void box_z_product(hipStream_t st, int n, int64_t nvec, double2 *d, const double2 *g) {
+  if (nvec <= 0) return;
+  with_side(n, [&](auto side) {
+    constexpr int N = decltype(side)::value;
+    k_box_z_mul_z<N><<<(unsigned)((nvec * N * N + 63) / 64), 64, 0, st>>>(nvec, d, g);
+  });
 }
 
 }  // namespace sgw

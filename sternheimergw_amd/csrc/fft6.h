@@ -19,5 +19,9 @@ void fft6_check(const Fft6Args &a, const char *who);
 // One matrix at a time on `st`; `scratch` holds num_g * n_p^3 elements.
 void invfft6_run(hipStream_t st, const Fft6Args &a, double2 *scratch);
 void fwfft6_run(hipStream_t st, const Fft6Args &a, double2 *scratch);
+// The parts of matrix m: invfft6 = front, then box_fft(n, n_p^3, f_m, -1);
+// fwfft6 = box_fft(n, n_p^3, f_m, +1), then back.
+void invfft6_front(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
+void fwfft6_back(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
 
 }  // namespace sgw
