@@ -232,13 +232,34 @@ extern "C" int sgw_sigma_prod(int32_t n, int32_t n_p, int32_t num_g, const int32
     const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
     DevBuf<double2> S((size_t)num_g * nr_p);
     KScope ks("sigma_prod", st);
-    // invfft6 (sigma.f90:487), the product with G (:499) and fwfft6 (:510);
-    // the r transforms' z passes of both and the product are one kernel
-    invfft6_front(st, a, S.p, 0);
-    box_fft_xy(st, n, nr_p, a.f, -1);
+    // invfft6 (sigma.f90:487), the product with G (:499) and fwfft6 (:510).
+    // Along r: the first xy pass builds its planes from the rows S[g][r']
+    // (no zero-filled matrix in HBM), the z passes of both transforms and
+    // the product are one kernel, and the last xy pass keeps only the rows'
+    // positions.  The plane lists: the G vectors of each z plane of the box.
+    std::vector<int32_t> nl(num_g);
+    SGW_HIP(hipMemcpyAsync(nl.data(), d_nl, sizeof(int32_t) * num_g, hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipStreamSynchronize(st));
+    for (int g = 0; g < num_g; ++g)
+      SGW_REQUIRE(nl[g] >= 0 && nl[g] < nr, SGW_E_ARG, "sigma_prod: box index out of range");
+    std::vector<int32_t> start(n + 1, 0), gl(num_g), pos(num_g);
+    for (int g = 0; g < num_g; ++g) ++start[nl[g] / (n * n) + 1];
+    for (int z = 0; z < n; ++z) start[z + 1] += start[z];
+    std::vector<int32_t> fill(start.begin(), start.end() - 1);
+    for (int g = 0; g < num_g; ++g) {
+      const int k = fill[nl[g] / (n * n)]++;
+      gl[k] = g;
+      pos[k] = nl[g] % (n * n);
+    }
+    DevBuf<int32_t> d_start(n + 1), d_gl(num_g), d_pos(num_g);
+    d_start.upload(start.data(), n + 1, st);
+    d_gl.upload(gl.data(), num_g, st);
+    d_pos.upload(pos.data(), num_g, st);
+    invfft6_rows(st, a, S.p, 0);
+    box_xy_scatter(st, n, nr_p, S.p, d_start.p, d_gl.p, d_pos.p, a.f, -1);
     box_z_product(st, n, nr_p, a.f, reinterpret_cast<const double2 *>(d_green));
-    box_fft_xy(st, n, nr_p, a.f, +1);
-    fwfft6_back(st, a, S.p, 0);
+    box_xy_gather(st, n, nr_p, a.f, d_start.p, d_gl.p, d_pos.p, 1.0 / (double)nr, S.p, +1);
+    fwfft6_rows(st, a, S.p, 0);
     k_alpha_block<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
         num_g, num_g_p, nr, make_double2(alpha.re, alpha.im), a.f,
         reinterpret_cast<double2 *>(d_sigma), ld_sigma);                  // :519-521, :741

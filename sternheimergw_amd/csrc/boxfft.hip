@@ -83,6 +83,94 @@ __global__ __launch_bounds__(64) void k_box_z_mul_z(int64_t nvec, double2 *__res
   for (int i = 0; i < N; ++i) p[(int64_t)i * N * N] = x[i];
 }
 
+// xy pass of box r' = blockIdx.x / N, plane z = blockIdx.x % N, whose input
+// holds only the values S[g][r'] at the box positions of the plane's list
+// (pos[k], k in [start[z], start[z + 1])) and zeros elsewhere: the plane is
+// built in LDS, so the zero-filled box never passes through HBM.
+template <int N, int SIGN>
+__global__ __launch_bounds__(64) void k_box_xy_scatter(const double2 *__restrict__ S,
+                                                       int64_t nr_p,
+                                                       const int32_t *__restrict__ start,
+                                                       const int32_t *__restrict__ gl,
+                                                       const int32_t *__restrict__ pos,
+                                                       double2 *__restrict__ d) {
+  constexpr int P = N + 1;
+  __shared__ double2 s[N * P];
+  const int64_t box = blockIdx.x / N;
+  const int z = blockIdx.x % N;
+  double2 *g = d + (int64_t)blockIdx.x * N * N;
+  for (int e = threadIdx.x; e < N * P; e += 64) s[e] = make_double2(0, 0);
+  __syncthreads();
+  for (int k = start[z] + threadIdx.x; k < start[z + 1]; k += 64) {
+    const int e = pos[k];
+    s[(e / N) * P + e % N] = S[(int64_t)gl[k] * nr_p + box];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < N) {  // x line y = t
+    double2 x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = s[t * P + i];
+    reg_fft<N, SIGN>(x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[t * P + i] = x[i];
+  }
+  __syncthreads();
+  if (t < N) {  // y line x = t
+    double2 x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = s[i * P + t];
+    reg_fft<N, SIGN>(x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i * P + t] = x[i];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < N * N; e += 64) g[e] = s[(e / N) * P + e % N];
+}
+
+// xy pass of a plane whose result is wanted only at the plane's list
+// positions: S[g][r'] = scale * (transformed plane)[pos]; the plane is not
+// written back.
+template <int N, int SIGN>
+__global__ __launch_bounds__(64) void k_box_xy_gather(const double2 *__restrict__ d,
+                                                      int64_t nr_p,
+                                                      const int32_t *__restrict__ start,
+                                                      const int32_t *__restrict__ gl,
+                                                      const int32_t *__restrict__ pos,
+                                                      double scale, double2 *__restrict__ S) {
+  constexpr int P = N + 1;
+  __shared__ double2 s[N * P];
+  const int64_t box = blockIdx.x / N;
+  const int z = blockIdx.x % N;
+  const double2 *g = d + (int64_t)blockIdx.x * N * N;
+  for (int e = threadIdx.x; e < N * N; e += 64) s[(e / N) * P + e % N] = g[e];
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < N) {  // x line y = t
+    double2 x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = s[t * P + i];
+    reg_fft<N, SIGN>(x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[t * P + i] = x[i];
+  }
+  __syncthreads();
+  if (t < N) {  // y line x = t
+    double2 x[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) x[i] = s[i * P + t];
+    reg_fft<N, SIGN>(x);
+#pragma unroll
+    for (int i = 0; i < N; ++i) s[i * P + t] = x[i];
+  }
+  __syncthreads();
+  for (int k = start[z] + threadIdx.x; k < start[z + 1]; k += 64) {
+    const int e = pos[k];
+    const double2 v = s[(e / N) * P + e % N];
+    S[(int64_t)gl[k] * nr_p + box] = make_double2(scale * v.x, scale * v.y);
+  }
+}
+
 template <int N, int SIGN> void run(hipStream_t st, int64_t nvec, double2 *d) {
   if (nvec <= 0) return;
   k_box_xy<N, SIGN><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
@@ -132,6 +220,32 @@ void box_fft_xy(hipStream_t st, int n, int64_t nvec, double2 *d, int sign) {
       k_box_xy<N, 1><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
     else
       k_box_xy<N, -1><<<(unsigned)(nvec * N), 64, 0, st>>>(d);
+  });
+}
+
+void box_xy_scatter(hipStream_t st, int n, int64_t nvec, const double2 *S, const int32_t *start,
+                    const int32_t *gl, const int32_t *pos, double2 *d, int sign) {
+  if (nvec <= 0) return;
+  with_side(n, [&](auto side) {
+    constexpr int N = decltype(side)::value;
+    if (sign > 0)
+      k_box_xy_scatter<N, 1><<<(unsigned)(nvec * N), 64, 0, st>>>(S, nvec, start, gl, pos, d);
+    else
+      k_box_xy_scatter<N, -1><<<(unsigned)(nvec * N), 64, 0, st>>>(S, nvec, start, gl, pos, d);
+  });
+}
+
+void box_xy_gather(hipStream_t st, int n, int64_t nvec, const double2 *d, const int32_t *start,
+                   const int32_t *gl, const int32_t *pos, double scale, double2 *S, int sign) {
+  if (nvec <= 0) return;
+  with_side(n, [&](auto side) {
+    constexpr int N = decltype(side)::value;
+    if (sign > 0)
+      k_box_xy_gather<N, 1><<<(unsigned)(nvec * N), 64, 0, st>>>(d, nvec, start, gl, pos, scale,
+                                                                 S);
+    else
+      k_box_xy_gather<N, -1><<<(unsigned)(nvec * N), 64, 0, st>>>(d, nvec, start, gl, pos, scale,
+                                                                  S);
   });
 }
 

@@ -23,5 +23,9 @@ void fwfft6_run(hipStream_t st, const Fft6Args &a, double2 *scratch);
 // fwfft6 = box_fft(n, n_p^3, f_m, +1), then back.
 void invfft6_front(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
 void fwfft6_back(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
+// The r' sides alone: invfft6_front = invfft6_rows, then the zero-filled
+// box scatter of the rows; fwfft6_back = the box gather, then fwfft6_rows.
+void invfft6_rows(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
+void fwfft6_rows(hipStream_t st, const Fft6Args &a, double2 *scratch, int m);
 
 }  // namespace sgw

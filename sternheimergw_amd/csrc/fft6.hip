@@ -123,9 +123,8 @@ void fft6_check(const Fft6Args &a, const char *who) {
   SGW_REQUIRE(a.omega > 0.0, SGW_E_ARG, std::string(who) + ": the cell volume must be positive");
 }
 
-// invfft6 of matrix m up to its last pass: f(r, r') in G along r, r' in r'
-// space; the r transform is left to the caller
-void invfft6_front(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
+// invfft6 of matrix m, r' side only: S[g][r'] for the num_g rows
+void invfft6_rows(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
   const int n = a.n, n_p = a.n_p, num_g = a.num_g, num_g_p = a.num_g_p;
   const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
   double2 *f = a.f + (int64_t)m * nr * nr_p;
@@ -133,20 +132,35 @@ void invfft6_front(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
   k_rows_scatter<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
       num_g, num_g_p, nr, nr_p, a.nl_p, f, 1.0 / a.omega, s);
   box_fft(st, n_p, num_g, s, +1);
+}
+
+// fwfft6 of matrix m, r' side only, from S[g][r'] into the block of f
+void fwfft6_rows(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
+  const int n = a.n, n_p = a.n_p, num_g = a.num_g, num_g_p = a.num_g_p;
+  const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
+  double2 *f = a.f + (int64_t)m * nr * nr_p;
+  box_fft(st, n_p, num_g, s, -1);
+  k_rows_gather<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
+      num_g, num_g_p, nr, nr_p, a.nl_p, s, a.omega / (double)nr_p, f);
+}
+
+// invfft6 of matrix m up to its last pass: f(r, r') in G along r, r' in r'
+// space; the r transform is left to the caller
+void invfft6_front(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
+  const int64_t nr = (int64_t)a.n * a.n * a.n, nr_p = (int64_t)a.n_p * a.n_p * a.n_p;
+  double2 *f = a.f + (int64_t)m * nr * nr_p;
+  invfft6_rows(st, a, s, m);
   k_fill0<<<blocks(nr * nr_p), kEB, 0, st>>>(nr * nr_p, f);
-  k_cols_scatter<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, a.nl, s, f);
+  k_cols_scatter<<<blocks((int64_t)a.num_g * nr_p), kEB, 0, st>>>(a.num_g, nr, nr_p, a.nl, s, f);
 }
 
 // fwfft6 of matrix m from after its first pass (f already transformed along r)
 void fwfft6_back(hipStream_t st, const Fft6Args &a, double2 *s, int m) {
-  const int n = a.n, n_p = a.n_p, num_g = a.num_g, num_g_p = a.num_g_p;
-  const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
+  const int64_t nr = (int64_t)a.n * a.n * a.n, nr_p = (int64_t)a.n_p * a.n_p * a.n_p;
   double2 *f = a.f + (int64_t)m * nr * nr_p;
-  k_cols_gather<<<blocks((int64_t)num_g * nr_p), kEB, 0, st>>>(num_g, nr, nr_p, a.nl, f,
-                                                                1.0 / (double)nr, s);
-  box_fft(st, n_p, num_g, s, -1);
-  k_rows_gather<<<blocks((int64_t)num_g * num_g_p), kEB, 0, st>>>(
-      num_g, num_g_p, nr, nr_p, a.nl_p, s, a.omega / (double)nr_p, f);
+  k_cols_gather<<<blocks((int64_t)a.num_g * nr_p), kEB, 0, st>>>(a.num_g, nr, nr_p, a.nl, f,
+                                                                  1.0 / (double)nr, s);
+  fwfft6_rows(st, a, s, m);
 }
 
 void invfft6_run(hipStream_t st, const Fft6Args &a, double2 *s) {
