@@ -13,7 +13,7 @@ reference's seed-only test, which leaves some shifts above 1e-8).
 A step is one batch of all 8192 systems solved from the right-hand sides
 resident in HBM to the solutions resident in HBM.  By default the timed
 steps' batches go through the solver as one queue (20 steps: 5120 systems),
-512 at a time (slots refilled as systems converge,
+1024 at a time (slots refilled as systems converge,
 sgw_batch_solver_run_stream); the end of the queue, where the slow systems
 finish with few others beside them, weighs less the longer the queue;
 --stream 0 solves each batch in lock step.  After the timed steps every (system, shift) true
@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--shift-test", type=int, default=1,
                     help="1: a system stops when every shift's residual estimate passes the "
                          "threshold; 0: the reference's test on the seed residual alone")
-    ap.add_argument("--slots", type=int, default=512,
+    ap.add_argument("--slots", type=int, default=1024,
                     help="solver slots of the streamed solve (0: one batch, 256); more slots "
                          "put more vectors into every H-apply launch")
     ap.add_argument("--stream", type=int, default=1,
