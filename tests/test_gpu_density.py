@@ -51,3 +51,20 @@ def test_drho_parseval_48cubed():
     s = got.reshape(w.nshift, -1).sum(1)
     ref = do.box_sum(w, 0, x[:, :, :n], 1.0)
     assert np.allclose(s, ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+def test_drho_matches_oracle_48cubed():
+    """Element by element on the 48^3 box (the half-width-8 split kernel):
+    2 bands x 4 shifts of random coefficients on the k+q sphere, at 1e-12
+    of the largest element.  Catches misplaced values (an x/y transpose)
+    that the box sum above cannot see."""
+    w = cell.workload_c2(nq=1, nbnd=2)
+    print("drho 48^3 element-wise: 2 bands x 4 shifts", flush=True)
+    op = operators.PwOperator.from_workload(w)
+    n = int(w.npw[0])
+    rng = np.random.Generator(np.random.PCG64(17))
+    x = np.zeros((2, 4, w.ld), dtype=np.complex128)
+    x[:, :, :n] = rng.standard_normal((2, 4, n)) + 1j * rng.standard_normal((2, 4, n))
+    got = _gpu_drho(op, w, 0, torch.from_numpy(x).to(DEV), 2, 4, 0.5)
+    ref = do.drho(w, 0, x[:, :, :n], 0.5)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -187,3 +187,21 @@ def test_batched_multigroup_parity(multi_q):
         rel = np.linalg.norm(x[s, :, :n] - xo, axis=1) / np.linalg.norm(xo, axis=1)
         assert rel.max() <= 1e-10
         assert np.all(x[s, :, n:] == 0)
+
+
+def test_nan_rhs_reports_ierr2(c1):
+    """A NaN in one right-hand side goes through the plane-wave operator
+    (whose transforms are built with finite-math flags) and the solver
+    reports ierr 2 for that system (bicgstab.f90:264-267), while the other
+    systems of the batch take exactly the iterations and solutions of a
+    batch without it."""
+    op = operators.PwOperator.from_workload(c1)
+    rhs = c1.rhs.copy()
+    rhs[1, 5] = np.nan
+    x0, e0, it0 = op.solve_batched(c1.rhs, c1.sigma, group=c1.group, threshold=1e-10)
+    x, e, it = op.solve_batched(rhs, c1.sigma, group=c1.group, threshold=1e-10)
+    assert e[1] == 2 and np.all(e0 == 0), (e, e0)
+    keep = [0, 2, 3]
+    assert np.all(e[keep] == 0)
+    assert np.array_equal(it[keep], it0[keep])
+    assert np.array_equal(x[keep], x0[keep])
