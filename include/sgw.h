@@ -151,6 +151,24 @@ int sgw_operator_apply_device(sgw_operator *op, int32_t nvec, const int32_t *d_g
                               const sgw_complex *d_sigma, const sgw_complex *d_x, int64_t ldx,
                               sgw_complex *d_ax, void *stream);
 
+/* Size of the fixed per-device work array of sgw_operator_apply_device_noalloc. */
+#define SGW_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* sgw_operator_apply_device for plane-wave operators without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The work buffers (z-stick and
+ * plane transforms, projector partial sums, vector lists) are carved from
+ * one array of SGW_NOALLOC_SCRATCH_BYTES per device, which the library holds
+ * until the process ends; a batch needing more returns SGW_E_SHAPE (at most
+ * 1024 vectors; on the 48^3 box with N_pw ~ 1700 about 120 vectors fit).
+ * Every call on a device shares that array: calls that overlap on one device
+ * (other streams or threads) race.  The call waits for earlier work on
+ * `stream` before it starts and returns with the application queued. */
+int sgw_operator_apply_device_noalloc(sgw_operator *op, int32_t nvec, const int32_t *d_group,
+                                      const sgw_complex *d_sigma, const sgw_complex *d_x,
+                                      int64_t ldx, sgw_complex *d_ax, void *stream);
+
 /* select_solver on a device operator (group selects the k+q point). */
 int sgw_select_solver_op(const sgw_select_solver_config *config, sgw_operator *op,
                          int32_t group, const sgw_complex *bb, int64_t n,

@@ -91,6 +91,8 @@ SIGNATURES = [
                                      c_void_p]),
     ("sgw_operator_apply_device", c_int32, [c_void_p, c_int32, c_void_p, c_void_p, c_void_p,
                                             c_int64, c_void_p, c_void_p]),
+    ("sgw_operator_apply_device_noalloc", c_int32, [c_void_p, c_int32, c_void_p, c_void_p,
+                                                    c_void_p, c_int64, c_void_p, c_void_p]),
     ("sgw_select_solver_op", c_int32, [ctypes.POINTER(sgw_select_solver_config), c_void_p, c_int32,
                                        c_void_p, c_int64, c_void_p, c_int32, c_void_p]),
     ("sgw_batch_solver_create", c_void_p, [c_void_p, ctypes.POINTER(sgw_batch_config)]),

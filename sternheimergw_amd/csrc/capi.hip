@@ -275,6 +275,19 @@ int sgw_operator_apply_device(sgw_operator *op, int32_t nvec, const int32_t *d_g
   });
 }
 
+int sgw_operator_apply_device_noalloc(sgw_operator *op, int32_t nvec, const int32_t *d_group,
+                                      const sgw_complex *d_sigma, const sgw_complex *d_x,
+                                      int64_t ldx, sgw_complex *d_ax, void *stream) {
+  return guarded([&] {
+    SGW_REQUIRE(op && op->op, SGW_E_ARG, "operator_apply_noalloc: null operator");
+    auto *pw = dynamic_cast<PwOperator *>(op->op.get());
+    SGW_REQUIRE(pw != nullptr, SGW_E_ARG,
+                "operator_apply_noalloc: plane-wave operators only");
+    pw->apply_noalloc((hipStream_t)stream, nvec, d_group, D2(d_sigma), D2(d_x), ldx, D2(d_ax));
+    return 0;
+  });
+}
+
 int sgw_operator_apply(sgw_operator *op, int32_t nvec, const int32_t *group,
                        const sgw_complex *sigma, const sgw_complex *x, int64_t ldx,
                        sgw_complex *ax) {

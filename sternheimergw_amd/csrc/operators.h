@@ -48,12 +48,28 @@ struct PwDesc {
   const double *dcoef;     // [ngroup][nproj] (Ry)
 };
 
+// Device memory one plane-wave application works in beyond its inputs and
+// outputs: z-stick and plane buffers, the local-potential result, projector
+// partial sums and overlaps, and the per-group vector lists.
+struct PwWork {
+  double2 *T, *S, *loc, *part, *becp;
+  const int32_t *plan_vec, *plan_cnt;
+  int plan_max, plan_tiles;
+};
+
 class PwOperator : public Operator {
  public:
   explicit PwOperator(const PwDesc &desc);
   void apply(hipStream_t st, int nvec, const double2 *in, int64_t in_stride, double2 *out,
              int64_t out_stride, const double2 *d_sigma, const double2 *h_sigma,
              const int32_t *d_active, const int32_t *h_active, const int32_t *d_group) override;
+  // The same application with every work buffer carved from the library's
+  // fixed per-device scratch array (no device allocation): vectors at
+  // in + v ldx, results at out + v ldx.  Rejects batches whose work buffers
+  // exceed the array.  Calls that overlap on one device share the array and
+  // race.
+  void apply_noalloc(hipStream_t st, int nvec, const int32_t *d_group, const double2 *d_sigma,
+                     const double2 *in, int64_t ldx, double2 *out);
   int64_t group_n(int g) const override;
   int ngroups() const override { return ngroup_; }
   void prepare(const int32_t *h_group, int nvec) override;
@@ -97,6 +113,9 @@ class PwOperator : public Operator {
   // one of the usual ones for the box (SGW_PW_BAND=0 keeps the general kernel)
   bool band_planes_ = true;
   void ensure_scratch(int nvec);
+  void apply_work(hipStream_t st, int nvec, const double2 *in, int64_t in_stride, double2 *out,
+                  int64_t out_stride, const double2 *d_sigma, const int32_t *d_active,
+                  const int32_t *d_group, const PwWork &w);
   // vectors of each group: plan_vec_[g * plan_max_ + j], j < plan_cnt_[g]
   int plan_nvec_ = -1, plan_max_ = 0, plan_tiles_ = 0;
   std::vector<int32_t> plan_key_;

@@ -20,7 +20,15 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
+#include <string>
 #include <type_traits>
+
+// The fixed work area of PwOperator::apply_noalloc: one copy per device,
+// held until the process ends.  Host code never uses this name as an address
+// (on the host it is only a stand-in for the symbol); it takes the device
+// address from hipGetSymbolAddress once per device (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[SGW_NOALLOC_SCRATCH_BYTES];
 
 namespace sgw {
 
@@ -1355,10 +1363,19 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
                        double2 *out, int64_t out_stride, const double2 *d_sigma, const double2 *,
                        const int32_t *d_active, const int32_t *, const int32_t *d_group) {
   if (nvec <= 0) return;
-  KScope whole("happly", st);
   SGW_REQUIRE(d_group != nullptr || ngroup_ == 1, SGW_E_ARG,
               "pw operator: a group index per vector is required with several groups");
+  SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
+              "pw operator: prepare() was not called for this batch of vectors");
   ensure_scratch(nvec);
+  PwWork w{T_.p, S_.p, loc_.p, part_.p, becp_.p, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_};
+  apply_work(st, nvec, in, in_stride, out, out_stride, d_sigma, d_active, d_group, w);
+}
+
+void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t in_stride,
+                            double2 *out, int64_t out_stride, const double2 *d_sigma,
+                            const int32_t *d_active, const int32_t *d_group, const PwWork &wk) {
+  KScope whole("happly", st);
   PwDev d;
   d.n1 = n1_; d.n2 = n2_; d.n3 = n3_;
   d.M1 = M1_; d.M2 = M2_; d.nc1 = nc1_; d.nc2 = nc2_; d.nslot = nslot_;
@@ -1374,8 +1391,6 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   const size_t lds_fft = sizeof(double2) * (nslot_ + (size_t)nc1_ * n1_);
   const dim3 gcol((unsigned)((nslot_ + 63) / 64), nvec);
   const double2 *locv = nullptr;
-  SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
-              "pw operator: prepare() was not called for this batch of vectors");
   const int nchunk = (int)((ld + kChunk - 1) / kChunk);
   const int nred_x = (nproj_ * kVT + 63) / 64;
   const bool fused = fuse_stages_ && fft_mode_ == 0 && cubic &&
@@ -1384,18 +1399,18 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     // projector contraction becp = D proj^H x (a concurrent stream for it was
     // measured: no gain, the plane kernel already fills the device)
     KScope ks("pw_becp", st, 2);
-    k_pw_becp<<<dim3(nchunk, ngroup_, plan_tiles_), 256, 0, st>>>(
-        d, in, in_stride, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
-        part_.p);
-    k_pw_becp_reduce<<<dim3((unsigned)nred_x, ngroup_, plan_tiles_), 64, 0, st>>>(
-        d, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk, part_.p, becp_.p);
+    k_pw_becp<<<dim3(nchunk, ngroup_, wk.plan_tiles), 256, 0, st>>>(
+        d, in, in_stride, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, wk.plan_tiles, nchunk,
+        wk.part);
+    k_pw_becp_reduce<<<dim3((unsigned)nred_x, ngroup_, wk.plan_tiles), 64, 0, st>>>(
+        d, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, wk.plan_tiles, nchunk, wk.part, wk.becp);
     SGW_HIP(hipGetLastError());
   }
   auto launch_fft = [&](auto tag) {
     constexpr int NB = decltype(tag)::value;
     if (fused) {
       // no projectors: the part counts are zero and only the transforms run
-      const int tiles = nproj_ > 0 ? plan_tiles_ : 0;
+      const int tiles = nproj_ > 0 ? wk.plan_tiles : 0;
       const int64_t nbecp = (int64_t)nchunk * ngroup_ * tiles;
       // z transforms split over lane pairs: 32 columns per 64 lanes
       const bool zsplit = split_planes_;
@@ -1410,13 +1425,13 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
           with_band<NB>(band_planes_, M3_, [&](auto mb) {
             k_pw_becp_zinv<NB, true, decltype(mb)::value>
                 <<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
-                    d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_,
-                    tiles, nchunk, ngroup_, part_.p, T_.p, nvec);
+                    d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max,
+                    tiles, nchunk, ngroup_, wk.part, wk.T, nvec);
           });
         else
           k_pw_becp_zinv<NB, false, -1><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
-              d, in, in_stride, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles,
-              nchunk, ngroup_, part_.p, T_.p, nvec);
+              d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles,
+              nchunk, ngroup_, wk.part, wk.T, nvec);
       }
       {
         KScope ks("pw_reduce_plane", st, 2);
@@ -1435,8 +1450,8 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
             k_pw_reduce_split<NB, decltype(mb)::value, pl, w>
                 <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + pl - 1) / pl)), 64 * w,
                    sizeof(double2) * pl * (size_t)nc1_ * NB, st>>>(
-                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
-                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+                    d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk,
+                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec);
           };
           const bool band = band_planes_ && M1_ == M2_;
           if (split_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
@@ -1448,15 +1463,15 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>
                 <<<(unsigned)(nred + (int64_t)NB * ((nvec + 1) / 2)), 64, 2 * lds_fft, st>>>(
-                    d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk,
-                    ngroup_, nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+                    d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk,
+                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec);
             paired = true;
           }
         }
         if (!paired) {
           k_pw_reduce_plane<NB, false><<<(unsigned)(nred + (int64_t)NB * nvec), 64, lds_fft, st>>>(
-              d, d_group, d_active, plan_vec_.p, plan_cnt_.p, plan_max_, tiles, nchunk, ngroup_,
-              nred_x, part_.p, becp_.p, T_.p, S_.p, nvec);
+              d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk, ngroup_,
+              nred_x, wk.part, wk.becp, wk.T, wk.S, nvec);
         }
       }
       {
@@ -1464,40 +1479,40 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
         if (zsplit)
           with_band<NB>(band_planes_, M3_, [&](auto mb) {
             k_pw_zfwd_split<NB, decltype(mb)::value>
-                <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p);
+                <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, wk.S, d_group, d_active, wk.loc);
           });
         else
-          k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, nullptr, 0);
+          k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, wk.S, d_group, d_active, wk.loc, nullptr, 0);
       }
-      locv = loc_.p;
+      locv = wk.loc;
       return;
     }
     // fft_mode_: 0 FP64, 1 FP32, 2 per vector from the relax flags
     const int32_t *rl = fft_mode_ == 2 ? relax_ : nullptr;
-    float2 *T32 = reinterpret_cast<float2 *>(T_.p), *S32 = reinterpret_cast<float2 *>(S_.p);
+    float2 *T32 = reinterpret_cast<float2 *>(wk.T), *S32 = reinterpret_cast<float2 *>(wk.S);
     const bool run64 = fft_mode_ != 1, run32 = fft_mode_ != 0;
     {
       KScope ks("pw_zinv", st, 2);
-      if (run64) k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T_.p, rl, 0);
+      if (run64) k_pw_zinv<NB><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, wk.T, rl, 0);
       if (run32)
         k_pw_zinv<NB, float2><<<gcol, 64, 0, st>>>(d, in, in_stride, d_group, d_active, T32, rl, 1);
     }
     {
       KScope ks("pw_plane", st, 2);
       if (run64)
-        k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, T_.p, d_group, d_active,
-                                                                 S_.p, rl, 0);
+        k_pw_plane_fft<NB><<<dim3(NB, nvec), 64, lds_fft, st>>>(d, wk.T, d_group, d_active,
+                                                                 wk.S, rl, 0);
       if (run32)
         k_pw_plane_fft<NB, float2><<<dim3(NB, nvec), 64, lds_fft / 2, st>>>(
             d, T32, d_group, d_active, S32, rl, 1);
     }
     {
       KScope ks("pw_zfwd", st, 2);
-      if (run64) k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, S_.p, d_group, d_active, loc_.p, rl, 0);
+      if (run64) k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, wk.S, d_group, d_active, wk.loc, rl, 0);
       if (run32)
-        k_pw_zfwd<NB, float2><<<gcol, 64, 0, st>>>(d, S32, d_group, d_active, loc_.p, rl, 1);
+        k_pw_zfwd<NB, float2><<<gcol, 64, 0, st>>>(d, S32, d_group, d_active, wk.loc, rl, 1);
     }
-    locv = loc_.p;
+    locv = wk.loc;
   };
   if (cubic && n1_ == 24) {
     launch_fft(std::integral_constant<int, 24>{});
@@ -1515,7 +1530,7 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     SGW_REQUIRE(lds_plane <= 64 * 1024, SGW_E_ARG, "pw operator: FFT plane does not fit in LDS");
     KScope ks("pw_plane", st, 2);
     k_pw_plane<<<dim3(n3_, nvec), kBlock, lds_plane, st>>>(d, in, in_stride, d_group, d_active,
-                                                          S_.p);
+                                                          wk.S);
   }
   SGW_HIP(hipGetLastError());
   {
@@ -1525,10 +1540,10 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
     // H-apply on the 48^3 batch, 2 % faster than 64 or 16 plane waves per
     // 256 lanes and 2-5 % faster than 8 or 16 per 64 lanes
     constexpr int CH = kAsmChunk, NT = kAsmThreads;
-    k_pw_assemble2<CH, NT><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, plan_tiles_), NT,
+    k_pw_assemble2<CH, NT><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, wk.plan_tiles), NT,
                              lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
-                                        plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, nchunk,
-                                        becp_.p, S_.p, locv);
+                                        wk.plan_vec, wk.plan_cnt, wk.plan_max, wk.plan_tiles, nchunk,
+                                        wk.becp, wk.S, locv);
   }
   SGW_HIP(hipGetLastError());
 }
@@ -1658,6 +1673,115 @@ void PwOperator::prepare(const int32_t *h_group, int nvec) {
   SGW_HIP(hipDeviceSynchronize());
   plan_key_ = std::move(key);
   plan_nvec_ = nvec;
+}
+
+
+// ---------------------------------------------------------------------------
+// application without device allocation
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocMaxVec = 1024;
+constexpr int kNoAllocMaxGroups = 256;
+constexpr int kNoAllocMaxPlan = 16384;  // ngroup x (largest group's vector count)
+
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+unsigned char *noalloc_scratch() {
+  int dev = 0;
+  SGW_HIP(hipGetDevice(&dev));
+  SGW_REQUIRE(dev >= 0 && dev < kNoAllocMaxDevices, SGW_E_ARG,
+              "pw operator no-alloc: device index beyond the supported range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void *p = nullptr;
+    SGW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SGW_REQUIRE(p != nullptr, SGW_E_STATE, "pw operator no-alloc: scratch symbol has no address");
+    g_noalloc_ptr[dev] = p;
+  }
+  return static_cast<unsigned char *>(g_noalloc_ptr[dev]);
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+void PwOperator::apply_noalloc(hipStream_t st, int nvec, const int32_t *d_group,
+                               const double2 *d_sigma, const double2 *in, int64_t ldx,
+                               double2 *out) {
+  SGW_REQUIRE(nvec >= 1 && nvec <= kNoAllocMaxVec, SGW_E_ARG,
+              "pw operator no-alloc: nvec must be in [1, 1024]");
+  SGW_REQUIRE(in && out && d_sigma, SGW_E_ARG, "pw operator no-alloc: null argument");
+  SGW_REQUIRE(ldx >= ld, SGW_E_SHAPE, "pw operator no-alloc: ldx < operator ld");
+  SGW_REQUIRE(d_group != nullptr || ngroup_ == 1, SGW_E_ARG,
+              "pw operator no-alloc: a group index per vector is required with several groups");
+  SGW_REQUIRE(ngroup_ <= kNoAllocMaxGroups, SGW_E_ARG,
+              "pw operator no-alloc: more than 256 k+q groups");
+  unsigned char *base = noalloc_scratch();
+  // group of every vector; the wait also orders this call after earlier
+  // work on the stream, which may still read the scratch array
+  int32_t grp[kNoAllocMaxVec];
+  if (d_group)
+    SGW_HIP(hipMemcpyAsync(grp, d_group, sizeof(int32_t) * nvec, hipMemcpyDeviceToHost, st));
+  SGW_HIP(hipStreamSynchronize(st));
+  if (!d_group) std::fill(grp, grp + nvec, 0);
+  // vectors of each group, in increasing order: plan[g * maxper + j]
+  int32_t cnt[kNoAllocMaxGroups] = {};
+  for (int v = 0; v < nvec; ++v) {
+    SGW_REQUIRE(grp[v] >= 0 && grp[v] < ngroup_, SGW_E_ARG,
+                "pw operator no-alloc: group index out of range");
+    ++cnt[grp[v]];
+  }
+  int maxper = 1;
+  for (int g = 0; g < ngroup_; ++g) maxper = std::max<int>(maxper, cnt[g]);
+  SGW_REQUIRE((int64_t)ngroup_ * maxper + ngroup_ <= kNoAllocMaxPlan, SGW_E_SHAPE,
+              "pw operator no-alloc: vector lists exceed 16384 entries (groups x largest group)");
+  int32_t plan[kNoAllocMaxPlan];
+  std::fill(plan, plan + (size_t)ngroup_ * maxper, 0);
+  int32_t fill[kNoAllocMaxGroups] = {};
+  for (int v = 0; v < nvec; ++v) plan[(size_t)grp[v] * maxper + fill[grp[v]]++] = v;
+  std::copy(cnt, cnt + ngroup_, plan + (size_t)ngroup_ * maxper);
+  // carve the work buffers
+  const int tiles = (maxper + kVT - 1) / kVT;
+  const int nchunk = (int)((ld + kChunk - 1) / kChunk);
+  const size_t np = std::max(nproj_, 1);
+  const size_t b_plan = align256(sizeof(int32_t) * ((size_t)ngroup_ * maxper + ngroup_));
+  const size_t b_part = align256(sizeof(double2) * (size_t)ngroup_ * tiles * nchunk * np * kVT);
+  const size_t b_becp = align256(sizeof(double2) * (size_t)ngroup_ * tiles * np * kVT);
+  const size_t b_box = align256(sizeof(double2) * (size_t)nvec * nslot_ * n3_);
+  const size_t b_loc = align256(sizeof(double2) * (size_t)nvec * ld);
+  const size_t total = b_plan + b_part + b_becp + 2 * b_box + b_loc;
+  SGW_REQUIRE(total <= (size_t)SGW_NOALLOC_SCRATCH_BYTES, SGW_E_SHAPE,
+              "pw operator no-alloc: this batch needs " + std::to_string(total) +
+                  " bytes of work space, more than the fixed " +
+                  std::to_string((size_t)SGW_NOALLOC_SCRATCH_BYTES) +
+                  "-byte scratch array; use fewer vectors per call");
+  unsigned char *p = base;
+  int32_t *d_plan = reinterpret_cast<int32_t *>(p);
+  p += b_plan;
+  PwWork w{};
+  w.part = reinterpret_cast<double2 *>(p);
+  p += b_part;
+  w.becp = reinterpret_cast<double2 *>(p);
+  p += b_becp;
+  w.T = reinterpret_cast<double2 *>(p);
+  p += b_box;
+  w.S = reinterpret_cast<double2 *>(p);
+  p += b_box;
+  w.loc = reinterpret_cast<double2 *>(p);
+  w.plan_vec = d_plan;
+  w.plan_cnt = d_plan + (size_t)ngroup_ * maxper;
+  w.plan_max = maxper;
+  w.plan_tiles = tiles;
+  SGW_HIP(hipMemcpyAsync(d_plan, plan, sizeof(int32_t) * ((size_t)ngroup_ * maxper + ngroup_),
+                         hipMemcpyHostToDevice, st));
+  if (nproj_ == 0) SGW_HIP(hipMemsetAsync(w.becp, 0, b_becp, st));
+  // the host lists above live on this stack frame: the copy must be done
+  SGW_HIP(hipStreamSynchronize(st));
+  apply_work(st, nvec, in, ldx, out, ldx, d_sigma, nullptr, d_group, w);
 }
 
 }  // namespace sgw
