@@ -108,11 +108,14 @@ struct Scal {
 // rho_old = -omega rho_old.  `commit` completes L18 of the previous step
 // (alpha_old = alpha = alpha_new), which the shifted-scalar update leaves to
 // this step so that its per-shift threads can all read the old alpha.
-// `fresh` (streaming solves, first step of an iteration only): a system in its
-// first iteration has no previous step to complete; the flag is consumed here.
+// `fresh` (first step of an iteration only): a system in its first iteration
+// has no previous step to complete; the flag is consumed here.  `count`
+// (first step only) counts the iterations each system has begun.
 __device__ inline void bicg_s1_body(int s, int begin, int commit, Scal sc,
-                                    int32_t *__restrict__ fresh, double2 rho) {
+                                    int32_t *__restrict__ fresh, int32_t *__restrict__ count,
+                                    double2 rho) {
   if (begin) sc.rho_old[s] = cmul(cneg(sc.omega[s]), sc.rho_old[s]);
+  if (count) count[s] += 1;
   if (fresh) {
     if (fresh[s]) commit = 0;
     fresh[s] = 0;
@@ -131,14 +134,15 @@ __global__ __launch_bounds__(kBlock) void k_dotu_s1(const int32_t *__restrict__ 
                                                     const double2 *__restrict__ a, int64_t sa,
                                                     const double2 *__restrict__ b, int64_t n,
                                                     int begin, int commit, Scal sc,
-                                                    int32_t *__restrict__ fresh) {
+                                                    int32_t *__restrict__ fresh,
+                                                    int32_t *__restrict__ count) {
   __shared__ double2 lds[kBlock / 64];
   const int s = blockIdx.x;
   if (!active[s]) return;
   const double2 acc = block_dotu(a + s * sa, b + s * n, n, lds);
   if (threadIdx.x == 0) {
     sc.dot[s] = acc;
-    bicg_s1_body(s, begin, commit, sc, fresh, acc);
+    bicg_s1_body(s, begin, commit, sc, fresh, count, acc);
   }
 }
 
@@ -473,7 +477,12 @@ __global__ __launch_bounds__(NT) void k_mr_mgs(const int32_t *__restrict__ activ
 // shift_test = 1 every shifted residual must pass too, through its estimate
 // ||r^sigma|| = |1 / (theta^sigma phi^sigma)| ||r_seed|| (the collinearity of
 // Frommer's recurrences, bicgstab.f90:637-643), which can exceed the seed's.
-__global__ void k_check(int nsys, int iter, double thr, const int32_t *__restrict__ in,
+// `count` holds the iterations each system has begun; after the MR half
+// (final = 1) a system that has run max_iter of them without converging stops
+// with state 3, iterations max_iter + 1 (the reference's loop variable after
+// DO iter = 1, max_iter, bicgstab.f90:229-253).
+__global__ void k_check(int nsys, const int32_t *__restrict__ count, int max_iter, int final,
+                        double thr, const int32_t *__restrict__ in,
                         int32_t *__restrict__ out, const double *__restrict__ norm,
                         int32_t *__restrict__ iters, int32_t *__restrict__ state, Scal sc,
                         int nsh, int shift_test, double relax_thr, int32_t *__restrict__ relax) {
@@ -499,12 +508,16 @@ __global__ void k_check(int nsys, int iter, double thr, const int32_t *__restric
   int keep = 1;
   if (nr < thr) {
     keep = 0;
-    iters[s] = iter;
+    iters[s] = count[s];
     state[s] = 1;  // converged
   } else if (isnan(nr)) {
     keep = 0;
-    iters[s] = iter;
+    iters[s] = count[s];
     state[s] = 2;  // NaN: the reference would run to max_iter and report ierr 2
+  } else if (final && count[s] >= max_iter) {
+    keep = 0;
+    iters[s] = max_iter + 1;
+    state[s] = 3;  // out of iterations: ierr 1
   }
   out[s] = keep;
 }
@@ -587,7 +600,7 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
                                  Scal sc, const double2 *__restrict__ sigma_all, int nshift_all,
                                  double2 *__restrict__ sigma_seed, int32_t *__restrict__ active,
                                  int32_t *__restrict__ iters, int32_t *__restrict__ state,
-                                 int32_t *__restrict__ fresh) {
+                                 int32_t *__restrict__ fresh, int32_t *__restrict__ count) {
   // one thread per (pair, shifted system); the k = 0 thread also does the seed
   const int nshw = nsh > 0 ? nsh : 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -602,13 +615,7 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
   iters[s] = 0;
   state[s] = 0;
   fresh[s] = 1;
-}
-
-// slots whose system ran out of iterations stop (pairs as above)
-__global__ void k_deactivate(int npairs, const int32_t *__restrict__ pairs,
-                             int32_t *__restrict__ active) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < npairs) active[pairs[2 * p]] = 0;
+  count[s] = 0;
 }
 
 // x_out[q][k][ldo] <- X / SX of slot s for every pair (s, q)
@@ -690,7 +697,7 @@ struct BatchBicgstab::Impl {
   DevBuf<double2> scal_seed, scal_shift, mu, coef, gam;
   DevBuf<double2> sigma_seed;  // per system seed shift sigma_1
   DevBuf<double> norms;
-  DevBuf<int32_t> active, mr_active, iters, state;
+  DevBuf<int32_t> active, mr_active, iters, state, count;
   Scal sc{};
   Vecs vv{};
   std::vector<double2> h_sigma_seed;
@@ -703,11 +710,85 @@ struct BatchBicgstab::Impl {
   DevBuf<int32_t> fresh, slot_group, pairs_dev;
   int32_t *pinned_stream = nullptr;
 
+  // Iterations as a captured graph: `chunk` iterations are captured once per
+  // solve on the solver's own stream and replayed by one hipGraphLaunch each
+  // (the caller's stream may be the legacy default stream, which cannot be
+  // captured; the solve joins it through events).  Off while kernel timing
+  // is on, whose events cannot sit inside a graph.
+  int graph_mode = 0;  // SGW_SOLVER_GRAPH=1 replays captured iterations
+  int graph_chunk = 1; // SGW_GRAPH_CHUNK: iterations per graph launch (lock-step solves)
+  hipStream_t own = nullptr;
+  hipEvent_t join_ev[2] = {nullptr, nullptr};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_n = 0;
+
   ~Impl() {
+    drop_graph();
     if (pinned_active) (void)hipHostFree(pinned_active);
     if (pinned_stream) (void)hipHostFree(pinned_stream);
     for (auto &e : flag_ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto &e : join_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (own) (void)hipStreamDestroy(own);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    graph_n = 0;
+  }
+
+  // the stream a solve runs on: the caller's, or, with graphs, the solver's
+  // own stream after the caller's earlier work
+  hipStream_t begin_on(hipStream_t st, bool graphs) {
+    if (!graphs) return st;
+    if (!own) {
+      SGW_HIP(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+      SGW_HIP(hipEventCreateWithFlags(&join_ev[0], hipEventDisableTiming));
+      SGW_HIP(hipEventCreateWithFlags(&join_ev[1], hipEventDisableTiming));
+    }
+    SGW_HIP(hipEventRecord(join_ev[0], st));
+    SGW_HIP(hipStreamWaitEvent(own, join_ev[0], 0));
+    return own;
+  }
+  // the caller's stream continues after the solve's work
+  void end_on(hipStream_t st, hipStream_t ws) {
+    drop_graph();
+    if (ws == st) return;
+    SGW_HIP(hipEventRecord(join_ev[1], ws));
+    SGW_HIP(hipStreamWaitEvent(st, join_ev[1], 0));
+  }
+  bool graphs_wanted(const Operator &op) const {
+    return graph_mode && !op.host_side() && ktimer_level() == 0;
+  }
+
+  // n iterations on ws: eagerly, or by replaying the graph of n iterations
+  // (captured at the first call of the solve)
+  void run(Operator &op, hipStream_t ws, bool graphs, int n, int max_iter, double thr) {
+    if (!graphs) {
+      for (int i = 0; i < n; ++i) iterate(op, ws, max_iter, thr);
+      return;
+    }
+    if (gexec && graph_n != n) drop_graph();
+    if (!gexec) {
+      SGW_HIP(hipStreamBeginCapture(ws, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int i = 0; i < n; ++i) iterate(op, ws, max_iter, thr);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(ws, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      SGW_HIP(hipStreamEndCapture(ws, &graph));
+      SGW_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      graph_n = n;
+    }
+    SGW_HIP(hipGraphLaunch(gexec, ws));
   }
 
   Impl(int nsys_, int nshift_, int l_, int64_t ld_)
@@ -717,6 +798,9 @@ struct BatchBicgstab::Impl {
       const int g = std::atoi(c);
       if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16) shift_groups = g;
     }
+    if (const char *c = std::getenv("SGW_SOLVER_GRAPH")) graph_mode = std::atoi(c) != 0;
+    if (const char *c = std::getenv("SGW_GRAPH_CHUNK"))
+      graph_chunk = std::max(1, std::min(16, std::atoi(c)));
     SGW_REQUIRE(nshift >= 1, SGW_E_ARG, "bicgstab: need at least one shift");
     const size_t S = nsys;
     R.alloc(S * (l + 1) * ld);
@@ -741,6 +825,8 @@ struct BatchBicgstab::Impl {
     mr_active.alloc(S);
     iters.alloc(S);
     state.alloc(S);
+    count.alloc(S);
+    fresh.alloc(S);
     double2 *p = scal_seed.p;
     sc.rho = p; sc.rho_old = p + S; sc.alpha = p + 2 * S; sc.alpha_old = p + 3 * S;
     sc.beta = p + 4 * S; sc.omega = p + 5 * S; sc.dot = p + 6 * S; sc.alpha_new = p + 7 * S;
@@ -779,16 +865,18 @@ struct BatchBicgstab::Impl {
   DevBuf<int32_t> relax;
 
   // ||r_0|| test on the systems of `in`; survivors go to `out`
-  void check(hipStream_t st, int iter, double thr, int32_t *in, int32_t *out) {
+  void check(hipStream_t st, int max_iter, int final, double thr, int32_t *in, int32_t *out) {
     k_norm<<<nsys, kBlock, 0, st>>>(nsys, in, R.p, (l + 1) * ld, ld, norms.p);
-    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, iter, thr, in, out, norms.p, iters.p,
-                                               state.p, sc, nsh, shift_test, relax_thr,
-                                               relax_thr > 0 ? relax.p : nullptr);
+    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, count.p, max_iter, final, thr, in, out,
+                                               norms.p, iters.p, state.p, sc, nsh, shift_test,
+                                               relax_thr, relax_thr > 0 ? relax.p : nullptr);
   }
 
-  // One outer iteration (bicgstab.f90:229-247) on every active system.
-  // `fresh` marks systems in their first iteration (streaming), else null.
-  void iterate(Operator &op, hipStream_t st, int iter, double threshold, int32_t *fresh) {
+  // One outer iteration (bicgstab.f90:229-247) on every active system.  Its
+  // launches do not depend on the iteration number: the `fresh` flags mark
+  // systems in their first iteration and `count` their iterations, so the
+  // same sequence can be captured once and replayed.
+  void iterate(Operator &op, hipStream_t st, int max_iter, double threshold) {
     const int S = nsys;
     const dim3 gv((unsigned)((ld + kBlock - 1) / kBlock), S);
     const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
@@ -798,8 +886,8 @@ struct BatchBicgstab::Impl {
       {
         KScope ks_("dotu", st, 2);
         k_dotu_s1<<<S, kBlock, 0, st>>>(active.p, R.p + j * ld, sstride, RT.p, ld, j == 0 ? 1 : 0,
-                                        (iter > 1 || j > 0) ? 1 : 0, sc,
-                                        j == 0 ? fresh : nullptr);
+                                        1, sc, j == 0 ? fresh.p : nullptr,
+                                        j == 0 ? count.p : nullptr);
       }
       {
         KScope ks_("bicg_v1", st, 2);
@@ -817,7 +905,7 @@ struct BatchBicgstab::Impl {
       }
       op_apply(op, st, R.p + j * ld, R.p + (j + 1) * ld);       // L24
     }
-    check(st, iter, threshold, active.p, mr_active.p);
+    check(st, max_iter, 0, threshold, active.p, mr_active.p);
     // ---- mr_part (Algorithm 4) on the systems still running
     {
       KScope ks_("mr_mgs", st, 2);
@@ -834,7 +922,7 @@ struct BatchBicgstab::Impl {
       KScope ks_("mr_seed", st, 2);
       launch_l<MrSeed>(l, gv, dim3(kBlock), st, (const int32_t *)mr_active.p, vv, sc);
     }
-    check(st, iter, threshold, mr_active.p, active.p);
+    check(st, max_iter, 1, threshold, mr_active.p, active.p);
   }
 };
 
@@ -842,8 +930,8 @@ BatchBicgstab::BatchBicgstab(int nsys, int nshift, int lmax, int64_t ld)
     : impl_(new Impl(nsys, nshift, lmax, ld)) {}
 BatchBicgstab::~BatchBicgstab() { delete impl_; }
 
-void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, int64_t rhs_ld,
-                          const double2 *d_sigma, const double2 *h_sigma,
+void BatchBicgstab::solve(Operator &op, hipStream_t caller_st, const double2 *d_rhs,
+                          int64_t rhs_ld, const double2 *d_sigma, const double2 *h_sigma,
                           const int32_t *d_group, double threshold, int max_iter,
                           double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
                           int check_every, int shift_test, double relax_thr) {
@@ -861,6 +949,18 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
       if (pw) pw->set_fft_precision(mode, flags);
     }
   } guard{nullptr, 0, nullptr};
+  const bool graphs = m.graphs_wanted(op);
+  hipStream_t st = m.begin_on(caller_st, graphs);
+  struct Join {
+    Impl &m;
+    hipStream_t caller, ws;
+    ~Join() {
+      try {
+        m.end_on(caller, ws);
+      } catch (...) {
+      }
+    }
+  } join{m, caller_st, st};
   if (m.relax_thr > 0) {
     if (m.relax.n < (size_t)S) m.relax.alloc(S);
     m.relax.zero(st);
@@ -886,8 +986,10 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   m.sigma_seed.upload(m.h_sigma_seed.data(), S, st);
   m.h_active.assign(S, 1);
   m.active.upload(m.h_active.data(), S, st);
+  m.fresh.upload(m.h_active.data(), S, st);  // every system starts its first iteration
   SGW_HIP(hipMemsetAsync(m.iters.p, 0, sizeof(int32_t) * S, st));
   SGW_HIP(hipMemsetAsync(m.state.p, 0, sizeof(int32_t) * S, st));
+  SGW_HIP(hipMemsetAsync(m.count.p, 0, sizeof(int32_t) * S, st));
   m.R.zero(st);
   m.U.zero(st);
   m.X.zero(st);
@@ -897,28 +999,30 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   k_init_vectors<<<gv, kBlock, 0, st>>>(S, m.vv, d_rhs, rhs_ld);
   k_init_scalars<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, l, m.sc, d_sigma, m.nshift);
   const bool host_op = op.host_side();
-  int iter = 0;
-  int last_iter = 0;
-  for (iter = 1; iter <= max_iter; ++iter) {
-    m.iterate(op, st, iter, threshold, nullptr);
-    last_iter = iter;
+  // every system stops by itself after max_iter iterations (k_check), so the
+  // loop ends once no flag is left; the bound is a guard
+  const int chunk = graphs ? m.graph_chunk : 1;
+  const int64_t cap = (int64_t)max_iter + chunk + 1;
+  int64_t done_iters = 0;
+  for (int64_t it = 1; done_iters < cap; ++it) {
+    m.run(op, st, graphs, chunk, max_iter, threshold);
+    done_iters += chunk;
     if (host_op) {  // host operators read the active flags on every call
       m.sync_active(st);
       bool any = false;
       for (int s = 0; s < S; ++s) any |= (m.h_active[s] != 0);
       if (!any) break;
     } else {
-      // Lagged test: copy this iteration's flags to pinned memory and wait
-      // only for the previous iteration's copy, so the next iteration is
-      // already queued while the host looks.  An iteration enqueued after
-      // every system has stopped is a no-op (all kernels skip inactive
-      // systems).
-      const int slot = iter & 1;
+      // Lagged test: copy this launch's flags to pinned memory and wait only
+      // for the previous launch's copy, so the next iterations are already
+      // queued while the host looks.  Iterations enqueued after every system
+      // has stopped are no-ops (all kernels skip inactive systems).
+      const int slot = (int)(it & 1);
       SGW_HIP(hipMemcpyAsync(m.pinned_active + (size_t)slot * S, m.active.p,
                              sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
       SGW_HIP(hipEventRecord(m.flag_ev[slot], st));
-      const bool last = iter == max_iter;
-      if (iter >= 2 || last) {
+      const bool last = done_iters >= cap;
+      if (it >= 2 || last) {
         const int prev = last ? slot : (slot ^ 1);
         SGW_HIP(hipEventSynchronize(m.flag_ev[prev]));
         bool any = false;
@@ -928,8 +1032,8 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
       }
     }
   }
-  (void)last_iter;
-  // systems still active ran out of iterations: ierr 1 (bicgstab.f90:249-253)
+  (void)check_every;
+  // systems that ran out of iterations: ierr 1 (bicgstab.f90:249-253)
   std::vector<int32_t> st_h(S), it_h(S);
   SGW_HIP(hipMemcpyAsync(st_h.data(), m.state.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
   SGW_HIP(hipMemcpyAsync(it_h.data(), m.iters.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
@@ -948,7 +1052,7 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   for (int s = 0; s < S; ++s) {
     int ierr = 0;
     int its = it_h[s];
-    if (m.h_active[s] || st_h[s] == 2) {
+    if (m.h_active[s] || st_h[s] == 2 || st_h[s] == 3) {
       ierr = 1;
       if (m.h_active[s]) its = max_iter + 1;
     }
@@ -958,7 +1062,7 @@ void BatchBicgstab::solve(Operator &op, hipStream_t st, const double2 *d_rhs, in
   }
 }
 
-void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const double2 *d_rhs,
+void BatchBicgstab::solve_stream(Operator &op, hipStream_t caller_st, int nq, const double2 *d_rhs,
                                  int64_t rhs_ld, const double2 *d_sigma, const double2 *h_sigma,
                                  const int32_t *h_group, double threshold, int max_iter,
                                  double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
@@ -966,6 +1070,18 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
   Impl &m = *impl_;
   const int S = m.nsys;
   SGW_REQUIRE(!op.host_side(), SGW_E_ARG, "bicgstab stream: device-resident operators only");
+  const bool graphs = m.graphs_wanted(op);
+  hipStream_t st = m.begin_on(caller_st, graphs);
+  struct Join {
+    Impl &m;
+    hipStream_t caller, ws;
+    ~Join() {
+      try {
+        m.end_on(caller, ws);
+      } catch (...) {
+      }
+    }
+  } join{m, caller_st, st};
   SGW_REQUIRE(nq >= 0 && h_group && rhs_ld >= m.ld, SGW_E_ARG,
               "bicgstab stream: need nq >= 0, host group indices and rhs_ld >= ld");
   m.shift_test = shift_test;
@@ -1002,7 +1118,6 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
       for (int k = 0; k < share[g]; ++k) slot_group[s++] = g;
     for (int t = s; t < S; ++t) slot_group[t] = slot_group[0];  // idle slots
   }
-  if (m.fresh.n < (size_t)S) m.fresh.alloc(S);
   if (m.slot_group.n < (size_t)S) m.slot_group.alloc(S);
   if (m.pairs_dev.n < (size_t)8 * S) m.pairs_dev.alloc((size_t)8 * S);
   if (!m.pinned_stream) {
@@ -1052,7 +1167,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
                                                                            rhs_ld);
     k_refill_scalars<<<grid1((int)pr.size() * std::max(m.nsh, 1), 64), 64, 0, st>>>(
         (int)pr.size(), dp, m.nsh, m.l, m.sc, d_sigma, m.nshift, m.sigma_seed.p, m.active.p,
-        m.iters.p, m.state.p, m.fresh.p);
+        m.iters.p, m.state.p, m.fresh.p, m.count.p);
   };
   // first fill
   std::vector<std::pair<int, int>> pr;
@@ -1067,7 +1182,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
   refill(pr, 3);  // list buffer 3 is next written at host iteration 3
   int running = (int)pr.size();
   for (int it = 1; running > 0; ++it) {
-    m.iterate(op, st, it, threshold, m.fresh.p);
+    m.run(op, st, graphs, 1, max_iter, threshold);
     const int slot = it & 1;
     int32_t *hf = h_flags + (size_t)slot * 3 * S;
     SGW_HIP(hipMemcpyAsync(hf, m.active.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
@@ -1081,26 +1196,17 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t st, int nq, const dou
     if (p < 1) continue;
     SGW_HIP(hipEventSynchronize(m.flag_ev[slot ^ 1]));
     const int32_t *fa = h_flags + (size_t)(slot ^ 1) * 3 * S;
-    std::vector<std::pair<int, int>> done, forced, fill;
+    std::vector<std::pair<int, int>> done, fill;
     for (int s = 0; s < S; ++s) {
       if (slot_q[s] < 0 || slot_start[s] > p) continue;
       const int q = slot_q[s];
-      if (!fa[s]) {  // converged or NaN at iteration <= p
-        if (h_iters) h_iters[q] = fa[2 * S + s] - slot_start[s] + 1;
+      if (!fa[s]) {  // converged, NaN or out of iterations (ierr 1) at iteration <= p
+        if (h_iters) h_iters[q] = fa[2 * S + s];
         if (h_ierr) h_ierr[q] = fa[S + s] == 1 ? 0 : 1;
-        done.push_back({s, q});
-      } else if (p - slot_start[s] + 1 >= max_iter) {  // ierr 1 (bicgstab.f90:249-253)
-        forced.push_back({s, q});
         done.push_back({s, q});
       }
     }
     if (done.empty()) continue;
-    if (!forced.empty()) {  // rare: stop them, then the buffer is free again
-      int32_t *dp = stage(forced, slot);
-      k_deactivate<<<grid1((int)forced.size(), 64), 64, 0, st>>>((int)forced.size(), dp,
-                                                                  m.active.p);
-      SGW_HIP(hipStreamSynchronize(st));
-    }
     {  // solutions of the finished systems
       int32_t *dp = stage(done, slot);
       k_gather_pairs<<<dim3(otiles, (unsigned)done.size(), m.nshift), kBlock, 0, st>>>(

@@ -59,3 +59,22 @@ def test_stream_max_iter_reports_ierr(c1):
     (xr, er, itr), (x, e, it) = _both(w, op, order, 2, 1e-14, max_iter=3)
     assert np.all(er == 1) and np.all(e == 1)
     assert np.all(it == 4)
+
+
+@pytest.mark.parametrize("chunk", ["1", "3"])
+def test_graph_replay_equals_eager(c1, monkeypatch, chunk):
+    """Iterations replayed from a captured hipGraph (SGW_SOLVER_GRAPH=1,
+    `chunk` iterations per launch) give the eager launches' solutions and
+    iteration counts bit for bit, lock-step and streamed (the captured
+    sequence does not depend on the iteration number)."""
+    w = c1
+    op = operators.PwOperator.from_workload(w)
+    order = np.arange(w.nsys)
+    eager = _both(w, op, order, 2, 1e-10)
+    monkeypatch.setenv("SGW_SOLVER_GRAPH", "1")
+    monkeypatch.setenv("SGW_GRAPH_CHUNK", chunk)
+    graph = _both(w, op, order, 2, 1e-10)
+    for (xe, ee, ie), (xg, eg, ig) in zip(eager, graph):
+        assert np.all(ee == 0) and np.array_equal(ee, eg)
+        assert np.array_equal(ie, ig)
+        assert np.array_equal(xe, xg)
