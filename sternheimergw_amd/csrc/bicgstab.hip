@@ -618,6 +618,86 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
   count[s] = 0;
 }
 
+// Compaction of running systems into the leading slots: pairs[2p] = source
+// slot, pairs[2p + 1] = destination slot (sources and destinations disjoint).
+// Every vector a later iteration reads moves; the residual snapshots are
+// rebuilt within each iteration and stay.
+__global__ void k_move_vectors(const int32_t *__restrict__ pairs, Vecs v) {
+  const int s = pairs[2 * blockIdx.y], d = pairs[2 * blockIdx.y + 1];
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  for (int i = 0; i <= v.l; ++i) {
+    v.r(d, i)[e] = v.r(s, i)[e];
+    v.u(d, i)[e] = v.u(s, i)[e];
+  }
+  v.x(d)[e] = v.x(s)[e];
+  v.RT[(int64_t)d * v.ld + e] = v.RT[(int64_t)s * v.ld + e];
+  for (int k = 0; k < v.nsh; ++k) {
+    v.su(d, k)[e] = v.su(s, k)[e];
+    v.sx(d, k)[e] = v.sx(s, k)[e];
+  }
+}
+
+// per-slot int arrays that move with a system (null entries are skipped)
+struct SlotFlags {
+  int32_t *f[8];
+};
+
+// Scalars and flags of the moved systems (pairs as above), one thread per
+// (pair, shifted system); the k = 0 thread also moves the seed's.  S is the
+// solver's slot count (the seed arrays' stride).
+__global__ void k_move_scalars(int npairs, const int32_t *__restrict__ pairs, int S, int nsh,
+                               int l, Scal sc, double2 *__restrict__ seed_all,
+                               double2 *__restrict__ sigma_seed, SlotFlags fl) {
+  const int nshw = nsh > 0 ? nsh : 1;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npairs * nshw) return;
+  const int p = t / nshw, k = t % nshw;
+  const int s = pairs[2 * p], d = pairs[2 * p + 1];
+  if (nsh > 0) {
+    const int is = s * nsh + k, id = d * nsh + k;
+    sc.sigma[id] = sc.sigma[is];
+    sc.inv_phi_old[id] = sc.inv_phi_old[is];
+    sc.inv_phi[id] = sc.inv_phi[is];
+    sc.inv_phi_new[id] = sc.inv_phi_new[is];
+    sc.inv_theta[id] = sc.inv_theta[is];
+    for (int j = 0; j < l; ++j) {
+      sc.alpha_s[id * l + j] = sc.alpha_s[is * l + j];
+      sc.beta_s[id * l + j] = sc.beta_s[is * l + j];
+      sc.fa[id * l + j] = sc.fa[is * l + j];
+      sc.fb[id * l + j] = sc.fb[is * l + j];
+      sc.ia[id * l + j] = sc.ia[is * l + j];
+    }
+    const int nmu = l * (l + 1) / 2;
+    for (int j = 0; j < nmu; ++j) sc.mu[(int64_t)id * nmu + j] = sc.mu[(int64_t)is * nmu + j];
+    for (int j = 0; j < kMaxL + 2; ++j)
+      sc.coef[(int64_t)id * (kMaxL + 2) + j] = sc.coef[(int64_t)is * (kMaxL + 2) + j];
+  }
+  if (k != 0) return;
+  for (int q = 0; q < 8; ++q) seed_all[(int64_t)q * S + d] = seed_all[(int64_t)q * S + s];
+  for (int j = 0; j < l; ++j) {
+    sc.gamma[d * l + j] = sc.gamma[s * l + j];
+    sc.gamma_p[d * l + j] = sc.gamma_p[s * l + j];
+    sc.gamma_pp[d * l + j] = sc.gamma_pp[s * l + j];
+  }
+  sigma_seed[d] = sigma_seed[s];
+  for (int f = 0; f < 8; ++f)
+    if (fl.f[f]) fl.f[f][d] = fl.f[f][s];
+}
+
+// Outcome of the systems of pairs (slot s, system q): iterations, state and
+// the running flag by system.
+__global__ void k_retire(int npairs, const int32_t *__restrict__ pairs,
+                         const int32_t *__restrict__ iters, const int32_t *__restrict__ state,
+                         const int32_t *__restrict__ active, int32_t *__restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int s = pairs[2 * p], q = pairs[2 * p + 1];
+  out[3 * q] = iters[s];
+  out[3 * q + 1] = state[s];
+  out[3 * q + 2] = active[s];
+}
+
 // x_out[q][k][ldo] <- X / SX of slot s for every pair (s, q)
 __global__ void k_gather_pairs(const int32_t *__restrict__ pairs, Vecs v, double2 *__restrict__ out,
                                int64_t ldo, int64_t n) {
@@ -799,6 +879,8 @@ struct BatchBicgstab::Impl {
       if (g == 1 || g == 2 || g == 4 || g == 8 || g == 16) shift_groups = g;
     }
     if (const char *c = std::getenv("SGW_SOLVER_GRAPH")) graph_mode = std::atoi(c) != 0;
+    if (const char *c = std::getenv("SGW_COMPACT")) compact_mode = std::atoi(c) != 0;
+    ns = nsys_;
     if (const char *c = std::getenv("SGW_GRAPH_CHUNK"))
       graph_chunk = std::max(1, std::min(16, std::atoi(c)));
     SGW_REQUIRE(nshift >= 1, SGW_E_ARG, "bicgstab: need at least one shift");
@@ -827,6 +909,8 @@ struct BatchBicgstab::Impl {
     state.alloc(S);
     count.alloc(S);
     fresh.alloc(S);
+    slot_grp.alloc(S);
+    outcome.alloc(3 * S);
     double2 *p = scal_seed.p;
     sc.rho = p; sc.rho_old = p + S; sc.alpha = p + 2 * S; sc.alpha_old = p + 3 * S;
     sc.beta = p + 4 * S; sc.omega = p + 5 * S; sc.dot = p + 6 * S; sc.alpha_new = p + 7 * S;
@@ -854,8 +938,55 @@ struct BatchBicgstab::Impl {
   }
 
   void op_apply(Operator &op, hipStream_t st, const double2 *in, double2 *out) {
-    op.apply(st, nsys, in, (l + 1) * ld, out, (l + 1) * ld, sigma_seed.p, h_sigma_seed.data(),
+    op.apply(st, ns, in, (l + 1) * ld, out, (l + 1) * ld, sigma_seed.p, h_sigma_seed.data(),
              active.p, h_active.data(), group_dev);
+  }
+
+  // Slots in use: the leading ns of the nsys slots.  Lock-step solves on
+  // device operators compact their running systems into fewer leading slots
+  // as systems finish (SGW_COMPACT=0 keeps all), so that late iterations
+  // launch grids for the running systems only.
+  int ns = 0;
+  int compact_mode = 1;
+  DevBuf<int32_t> slot_grp, outcome;  // group per slot; (iters, state, active) per system
+
+  // pairs (a, b) through pinned staging buffer b of the stream buffers
+  int32_t *stage_pairs(hipStream_t st, const std::vector<std::pair<int, int>> &pr, int buf) {
+    int32_t *hp = pinned_stream + 6 * nsys + (size_t)buf * 2 * nsys;
+    for (size_t i = 0; i < pr.size(); ++i) {
+      hp[2 * i] = pr[i].first;
+      hp[2 * i + 1] = pr[i].second;
+    }
+    int32_t *dp = pairs_dev.p + (size_t)buf * 2 * nsys;
+    SGW_HIP(hipMemcpyAsync(dp, hp, sizeof(int32_t) * 2 * pr.size(), hipMemcpyHostToDevice, st));
+    return dp;
+  }
+  void ensure_stream_buffers() {
+    if (pairs_dev.n < (size_t)8 * nsys) pairs_dev.alloc((size_t)8 * nsys);
+    if (!pinned_stream)
+      SGW_HIP(hipHostMalloc((void **)&pinned_stream, sizeof(int32_t) * 14 * nsys,
+                            hipHostMallocDefault));
+  }
+  // solutions and outcomes of the systems in slots `pr` (slot, system)
+  void retire(hipStream_t st, const std::vector<std::pair<int, int>> &pr, int buf, double2 *d_x,
+              int64_t x_ld, int64_t nout) {
+    if (pr.empty()) return;
+    int32_t *dp = stage_pairs(st, pr, buf);
+    k_gather_pairs<<<dim3((unsigned)((nout + kBlock - 1) / kBlock), (unsigned)pr.size(), nshift),
+                     kBlock, 0, st>>>(dp, vv, d_x, x_ld, nout);
+    k_retire<<<grid1((int)pr.size(), 64), 64, 0, st>>>((int)pr.size(), dp, iters.p, state.p,
+                                                        active.p, outcome.p);
+  }
+  // move the systems of `mv` (source slot, destination slot)
+  void move(hipStream_t st, const std::vector<std::pair<int, int>> &mv, int buf) {
+    if (mv.empty()) return;
+    int32_t *dp = stage_pairs(st, mv, buf);
+    k_move_vectors<<<dim3((unsigned)((ld + kBlock - 1) / kBlock), (unsigned)mv.size()), kBlock, 0,
+                     st>>>(dp, vv);
+    SlotFlags fl{{active.p, mr_active.p, iters.p, state.p, count.p, fresh.p,
+                  relax_thr > 0 ? relax.p : nullptr, slot_grp.p}};
+    k_move_scalars<<<grid1((int)mv.size() * std::max(nsh, 1), 64), 64, 0, st>>>(
+        (int)mv.size(), dp, nsys, nsh, l, sc, scal_seed.p, sigma_seed.p, fl);
   }
 
   const int32_t *group_dev = nullptr;
@@ -866,8 +997,8 @@ struct BatchBicgstab::Impl {
 
   // ||r_0|| test on the systems of `in`; survivors go to `out`
   void check(hipStream_t st, int max_iter, int final, double thr, int32_t *in, int32_t *out) {
-    k_norm<<<nsys, kBlock, 0, st>>>(nsys, in, R.p, (l + 1) * ld, ld, norms.p);
-    k_check<<<grid1(nsys, 128), 128, 0, st>>>(nsys, count.p, max_iter, final, thr, in, out,
+    k_norm<<<ns, kBlock, 0, st>>>(ns, in, R.p, (l + 1) * ld, ld, norms.p);
+    k_check<<<grid1(ns, 128), 128, 0, st>>>(ns, count.p, max_iter, final, thr, in, out,
                                                norms.p, iters.p, state.p, sc, nsh, shift_test,
                                                relax_thr, relax_thr > 0 ? relax.p : nullptr);
   }
@@ -877,7 +1008,7 @@ struct BatchBicgstab::Impl {
   // systems in their first iteration and `count` their iterations, so the
   // same sequence can be captured once and replayed.
   void iterate(Operator &op, hipStream_t st, int max_iter, double threshold) {
-    const int S = nsys;
+    const int S = ns;
     const dim3 gv((unsigned)((ld + kBlock - 1) / kBlock), S);
     const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
     const int64_t sstride = (int64_t)(l + 1) * ld;
@@ -969,18 +1100,23 @@ void BatchBicgstab::solve(Operator &op, hipStream_t caller_st, const double2 *d_
   }
   SGW_REQUIRE(d_group != nullptr || op.ngroups() == 1, SGW_E_ARG,
               "bicgstab: a group index per system is required with several groups");
-  {  // every group index must address an operator group
-    std::vector<int32_t> hg(S, 0);
-    if (d_group) {
-      SGW_HIP(hipMemcpyAsync(hg.data(), d_group, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-      SGW_HIP(hipStreamSynchronize(st));
-    }
-    for (int s = 0; s < S; ++s)
-      SGW_REQUIRE(hg[s] >= 0 && hg[s] < op.ngroups(), SGW_E_ARG,
-                  "bicgstab: group index out of range");
-    op.prepare(hg.data(), S);
+  // every group index must address an operator group; slot s starts with system s
+  std::vector<int32_t> slot_grp(S, 0), slot_sys(S);
+  if (d_group) {
+    SGW_HIP(hipMemcpyAsync(slot_grp.data(), d_group, sizeof(int32_t) * S, hipMemcpyDeviceToHost,
+                           st));
+    SGW_HIP(hipStreamSynchronize(st));
   }
-  m.group_dev = d_group;
+  for (int s = 0; s < S; ++s) {
+    SGW_REQUIRE(slot_grp[s] >= 0 && slot_grp[s] < op.ngroups(), SGW_E_ARG,
+                "bicgstab: group index out of range");
+    slot_sys[s] = s;
+  }
+  m.ns = S;
+  op.prepare(slot_grp.data(), S);
+  m.slot_grp.upload(slot_grp.data(), S, st);
+  m.group_dev = m.slot_grp.p;
+  m.ensure_stream_buffers();
   m.h_sigma_seed.resize(S);
   for (int s = 0; s < S; ++s) m.h_sigma_seed[s] = h_sigma[(int64_t)s * m.nshift];
   m.sigma_seed.upload(m.h_sigma_seed.data(), S, st);
@@ -999,11 +1135,14 @@ void BatchBicgstab::solve(Operator &op, hipStream_t caller_st, const double2 *d_
   k_init_vectors<<<gv, kBlock, 0, st>>>(S, m.vv, d_rhs, rhs_ld);
   k_init_scalars<<<grid1(S, 64), 64, 0, st>>>(S, m.nsh, l, m.sc, d_sigma, m.nshift);
   const bool host_op = op.host_side();
+  const bool compact = m.compact_mode && !host_op;
+  const int64_t nout = std::min<int64_t>(x_ld, m.ld);
   // every system stops by itself after max_iter iterations (k_check), so the
   // loop ends once no flag is left; the bound is a guard
   const int chunk = graphs ? m.graph_chunk : 1;
   const int64_t cap = (int64_t)max_iter + chunk + 1;
   int64_t done_iters = 0;
+  bool skip_check = false;
   for (int64_t it = 1; done_iters < cap; ++it) {
     m.run(op, st, graphs, chunk, max_iter, threshold);
     done_iters += chunk;
@@ -1012,53 +1151,89 @@ void BatchBicgstab::solve(Operator &op, hipStream_t caller_st, const double2 *d_
       bool any = false;
       for (int s = 0; s < S; ++s) any |= (m.h_active[s] != 0);
       if (!any) break;
-    } else {
-      // Lagged test: copy this launch's flags to pinned memory and wait only
-      // for the previous launch's copy, so the next iterations are already
-      // queued while the host looks.  Iterations enqueued after every system
-      // has stopped are no-ops (all kernels skip inactive systems).
-      const int slot = (int)(it & 1);
-      SGW_HIP(hipMemcpyAsync(m.pinned_active + (size_t)slot * S, m.active.p,
-                             sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-      SGW_HIP(hipEventRecord(m.flag_ev[slot], st));
-      const bool last = done_iters >= cap;
-      if (it >= 2 || last) {
-        const int prev = last ? slot : (slot ^ 1);
-        SGW_HIP(hipEventSynchronize(m.flag_ev[prev]));
-        bool any = false;
-        const int32_t *pa = m.pinned_active + (size_t)prev * S;
-        for (int s = 0; s < S; ++s) any |= (pa[s] != 0);
-        if (!any) break;
-      }
+      continue;
     }
+    // Lagged test: copy this launch's flags to pinned memory and wait only
+    // for the previous launch's copy, so the next iterations are already
+    // queued while the host looks.  Iterations enqueued after every system
+    // has stopped are no-ops (all kernels skip inactive systems).
+    const int slot = (int)(it & 1);
+    const int ns = m.ns;
+    SGW_HIP(hipMemcpyAsync(m.pinned_active + (size_t)slot * S, m.active.p, sizeof(int32_t) * ns,
+                           hipMemcpyDeviceToHost, st));
+    SGW_HIP(hipEventRecord(m.flag_ev[slot], st));
+    const bool last = done_iters >= cap;
+    if (it < 2 && !last) continue;
+    const int prev = last ? slot : (slot ^ 1);
+    SGW_HIP(hipEventSynchronize(m.flag_ev[prev]));
+    if (skip_check) {  // flags of the launch before a compaction: old slot order
+      skip_check = false;
+      continue;
+    }
+    const int32_t *pa = m.pinned_active + (size_t)prev * S;
+    int running = 0;
+    for (int s = 0; s < ns; ++s) running += pa[s] != 0;
+    if (!running) break;
+    if (!compact || running > ns / 2 || ns < 8) continue;
+    // Fewer than half the slots run: wait for the launch just queued (its
+    // flags are exact), retire the finished systems, move the running ones
+    // into the leading slots and continue on those.
+    SGW_HIP(hipEventSynchronize(m.flag_ev[slot]));
+    const int32_t *pc = m.pinned_active + (size_t)slot * S;
+    std::vector<std::pair<int, int>> done, mv;
+    int keep = 0;
+    for (int s = 0; s < ns; ++s) keep += pc[s] != 0;
+    if (keep == 0) break;
+    for (int s = 0; s < ns; ++s)
+      if (!pc[s]) done.push_back({s, slot_sys[s]});
+    int dst = 0;
+    for (int s = keep; s < ns; ++s) {
+      if (!pc[s]) continue;
+      while (pc[dst]) ++dst;  // a free slot below keep (one exists for every running one above)
+      mv.push_back({s, dst});
+      slot_sys[dst] = slot_sys[s];
+      slot_grp[dst] = slot_grp[s];
+      ++dst;
+    }
+    m.retire(st, done, 0, d_x, x_ld, nout);
+    m.move(st, mv, 1);
+    m.ns = keep;
+    op.prepare(slot_grp.data(), keep);
+    SGW_HIP(hipStreamSynchronize(st));  // the staging buffers are free again
+    m.drop_graph();  // its launches were sized for the old slot count
+    skip_check = true;
   }
   (void)check_every;
-  // systems that ran out of iterations: ierr 1 (bicgstab.f90:249-253)
-  std::vector<int32_t> st_h(S), it_h(S);
-  SGW_HIP(hipMemcpyAsync(st_h.data(), m.state.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-  SGW_HIP(hipMemcpyAsync(it_h.data(), m.iters.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
-  SGW_HIP(hipMemcpyAsync(m.h_active.data(), m.active.p, sizeof(int32_t) * S,
+  // the systems still in slots: solutions and outcomes
+  {
+    std::vector<std::pair<int, int>> rest;
+    if (host_op)
+      for (int s = 0; s < S; ++s) rest.push_back({s, s});
+    else
+      for (int s = 0; s < m.ns; ++s) rest.push_back({s, slot_sys[s]});
+    m.retire(st, rest, 0, d_x, x_ld, nout);
+  }
+  std::vector<int32_t> out_h((size_t)3 * S);
+  SGW_HIP(hipMemcpyAsync(out_h.data(), m.outcome.p, sizeof(int32_t) * 3 * S,
                          hipMemcpyDeviceToHost, st));
-  // outputs
-  const int64_t nout = std::min<int64_t>(x_ld, m.ld);
-  k_gather_x<<<dim3((unsigned)((nout + kBlock - 1) / kBlock), S, m.nshift), kBlock, 0, st>>>(
-      m.vv, d_x, x_ld, nout);
   DevBuf<int32_t> nanflag(S);
   nanflag.zero(st);
   k_nan_scan<<<S, kBlock, 0, st>>>(S, m.nshift, d_x, x_ld, nout, nanflag.p);
   std::vector<int32_t> nan_h(S);
   SGW_HIP(hipMemcpyAsync(nan_h.data(), nanflag.p, sizeof(int32_t) * S, hipMemcpyDeviceToHost, st));
   SGW_HIP(hipStreamSynchronize(st));
-  for (int s = 0; s < S; ++s) {
+  for (int q = 0; q < S; ++q) {
+    const int its = out_h[3 * q], sts = out_h[3 * q + 1], act = out_h[3 * q + 2];
     int ierr = 0;
-    int its = it_h[s];
-    if (m.h_active[s] || st_h[s] == 2 || st_h[s] == 3) {
+    int iters_q = its;
+    // still running, NaN or out of iterations: ierr 1 (bicgstab.f90:249-253)
+    if (act || sts == 2 || sts == 3) {
       ierr = 1;
-      if (m.h_active[s]) its = max_iter + 1;
+      if (act) iters_q = max_iter + 1;
     }
-    if (nan_h[s]) ierr = 2;  // bicgstab.f90:264-267
-    if (h_ierr) h_ierr[s] = ierr;
-    if (h_iters) h_iters[s] = its;
+    if (nan_h[q]) ierr = 2;  // bicgstab.f90:264-267
+    if (h_ierr) h_ierr[q] = ierr;
+    if (h_iters) h_iters[q] = iters_q;
   }
 }
 
@@ -1119,6 +1294,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t caller_st, int nq, co
     for (int t = s; t < S; ++t) slot_group[t] = slot_group[0];  // idle slots
   }
   if (m.slot_group.n < (size_t)S) m.slot_group.alloc(S);
+  m.ns = S;
   if (m.pairs_dev.n < (size_t)8 * S) m.pairs_dev.alloc((size_t)8 * S);
   if (!m.pinned_stream) {
     SGW_HIP(hipHostMalloc((void **)&m.pinned_stream, sizeof(int32_t) * 14 * S,

@@ -14,6 +14,7 @@
 #include "ktimer.h"
 #include "regfft.h"
 #include "splitfft.h"
+#include "tune.h"
 
 #include <algorithm>
 #include <cmath>
@@ -568,9 +569,10 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
 //      of plane p, m1 -> x, times V(x, y, z), x -> m1 (nc1 frequencies kept);
 //   Y' as Y, y -> m2, into the sphere columns S[v][z][c].
 // With PL N a multiple of 32 W (4 planes of 48 on 3 wavefronts) every lane
-// pair works in every X round.  LDS holds P[PL][nc1][N]; the column values
-// T[PL][nslot] are staged into its start and read into registers before the
-// Y stage overwrites them.
+// pair works in every X round.  LDS holds P[PL][nc1][N + 1] (rows padded by
+// one value, so that the lanes of the Y stages, one row each, fall on
+// different banks); the column values T[PL][nslot] are staged into its start
+// and read into registers before the Y stage overwrites them.
 // MB >= 0: the sphere's half-widths M1 = M2 = MB are compile-time constants.
 // Every transform input outside |m| <= MB is then a literal zero and every
 // output outside it is never stored, so the compiler drops the butterflies
@@ -584,6 +586,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
                                         double2 *__restrict__ S, int z, int v0, int nvec,
                                         double2 *smem) {
   constexpr int H = N / 2, Q = N / 4, NP = 32 * W;  // NP lane pairs
+  constexpr int PN = N + 1;  // row stride of P: rows of N values would share LDS banks
   const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
   const int nslot = d.nslot, M2 = MB >= 0 ? MB : d.M2;
   const int t = threadIdx.x, r = t >> 1;
@@ -595,7 +598,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
     any |= live[p];
   }
   if (!any) return;
-  double2 *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][N]
+  double2 *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][PN]
 #pragma unroll
   for (int p = 0; p < PL; ++p)  // plane p staged by wavefront p mod W
     if ((t >> 6) == p % W && live[p])
@@ -615,7 +618,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
   __syncthreads();  // every column value is in registers before P overwrites T
   if (yrow) {  // Y inverse of row a
     split_dit_sel<N, +1>(b, odd);
-    double2 *Pr = P + (yp * nc1 + a) * N + (odd ? Q : 0);
+    double2 *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       Pr[j] = b[j];
@@ -635,11 +638,11 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
       __asm__ volatile("" : "+v"(y)::"memory");
     else
       __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
-    double2 *Pp = P + p * nc1 * N + y;
+    double2 *Pp = P + p * nc1 * PN + y;
 #pragma unroll
     for (int n = 0; n < H; ++n) {
       const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * N] : mk(0, 0);
+      b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * PN] : mk(0, 0);
     }
     split_dit_sel<N, +1>(b, odd);
     const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
@@ -653,13 +656,13 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      if (m >= -M1 && m <= M1) Pp[(m + M1) * N] = b[k];
+      if (m >= -M1 && m <= M1) Pp[(m + M1) * PN] = b[k];
     }
   }
   __syncthreads();
   if (yrow) {  // Y forward of row a into the sphere columns
     const int v = v0 + yp;
-    const double2 *Pr = P + (yp * nc1 + a) * N + (odd ? Q : 0);
+    const double2 *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       b[j] = Pr[j];
@@ -1443,21 +1446,38 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
         if constexpr (NB <= 48) {
           // split transforms over lane pairs (LDS: P of the block's planes only)
           constexpr int PL = SplitShape<NB>::PL, W = SplitShape<NB>::W;
-          const size_t lds_split = sizeof(double2) * PL * (size_t)nc1_ * NB;
+          const size_t lds_split = sizeof(double2) * PL * (size_t)nc1_ * (NB + 1);
           auto launch_split = [&](auto shape, auto mb) {
             constexpr int pl = decltype(shape)::value / 8, w = decltype(shape)::value % 8;
             const int64_t nred_s = (int64_t)((nred_x + w - 1) / w) * ngroup_ * tiles;
             k_pw_reduce_split<NB, decltype(mb)::value, pl, w>
                 <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + pl - 1) / pl)), 64 * w,
-                   sizeof(double2) * pl * (size_t)nc1_ * NB, st>>>(
+                   sizeof(double2) * pl * (size_t)nc1_ * (NB + 1), st>>>(
                     d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk,
                     ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec);
           };
           const bool band = band_planes_ && M1_ == M2_;
           if (split_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
-            // compile-time sphere half-widths for the usual ones of this box
+            // compile-time sphere half-widths for the usual ones of this box;
+            // on the 48 box the autotuner also tries one plane on two
+            // wavefronts (the X stage in one round) and two planes on two
+            // for this size class of the batch
             with_band<NB>(band, M1_, [&](auto mb) {
-              launch_split(std::integral_constant<int, PL * 8 + W>{}, mb);
+              using I11 = std::integral_constant<int, PL * 8 + W>;
+              if constexpr (NB == 48) {
+                const size_t row = sizeof(double2) * (size_t)nc1_ * (NB + 1);
+                const int ncand = (nc1_ <= 32 && 2 * row <= 64 * 1024) ? 3 : 1;
+                const std::string key = "plane48/M" + std::to_string(M1_) + "/b" +
+                                        std::to_string((int)decltype(mb)::value) + "/n" +
+                                        std::to_string(size_class(nvec));
+                autotune(key, ncand, st, [&](int c) {
+                  if (c == 0) launch_split(I11{}, mb);
+                  else if (c == 1) launch_split(std::integral_constant<int, 1 * 8 + 2>{}, mb);
+                  else launch_split(std::integral_constant<int, 2 * 8 + 2>{}, mb);
+                });
+              } else {
+                launch_split(I11{}, mb);
+              }
             });
             paired = true;
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
@@ -1538,12 +1558,26 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
     KScope ks("pw_assemble", st, 2);
     // 32 plane waves x 8 vector groups per 256-lane block: measured per
     // H-apply on the 48^3 batch, 2 % faster than 64 or 16 plane waves per
-    // 256 lanes and 2-5 % faster than 8 or 16 per 64 lanes
-    constexpr int CH = kAsmChunk, NT = kAsmThreads;
-    k_pw_assemble2<CH, NT><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, wk.plan_tiles), NT,
-                             lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
-                                        wk.plan_vec, wk.plan_cnt, wk.plan_max, wk.plan_tiles, nchunk,
-                                        wk.becp, wk.S, locv);
+    // 256 lanes and 2-5 % faster than 8 or 16 per 64 lanes; the autotuner
+    // tries 16 and 64 as well for each size class of the batch (not when the
+    // output overwrites the input, which a second launch would read)
+    constexpr int NT = kAsmThreads;
+    auto launch_asm = [&](auto chunk) {
+      constexpr int CH = decltype(chunk)::value;
+      k_pw_assemble2<CH, NT><<<dim3((unsigned)((ld + CH - 1) / CH), ngroup_, wk.plan_tiles), NT,
+                               lds_asm, st>>>(d, in, in_stride, out, out_stride, d_sigma, d_active,
+                                              wk.plan_vec, wk.plan_cnt, wk.plan_max,
+                                              wk.plan_tiles, nchunk, wk.becp, wk.S, locv);
+    };
+    const bool disjoint = out + (int64_t)(nvec - 1) * out_stride + ld <= in ||
+                          in + (int64_t)(nvec - 1) * in_stride + ld <= out;
+    const std::string key = "assemble/n" + std::to_string(size_class(nvec)) + "/g" +
+                            std::to_string(ngroup_) + "/p" + std::to_string(nproj_);
+    autotune(key, disjoint ? 3 : 1, st, [&](int c) {
+      if (c == 0) launch_asm(std::integral_constant<int, kAsmChunk>{});
+      else if (c == 1) launch_asm(std::integral_constant<int, 16>{});
+      else launch_asm(std::integral_constant<int, 64>{});
+    });
   }
   SGW_HIP(hipGetLastError());
 }

@@ -1,0 +1,102 @@
+// Launch-shape autotuner (see tune.h).
+#include "tune.h"
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+
+namespace sgw {
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::string, int> g_choice;
+
+bool tuning_enabled() {
+  const char *c = std::getenv("SGW_AUTOTUNE");
+  return !(c && std::atoi(c) == 0);
+}
+
+// SGW_AUTOTUNE_FORCE=i runs candidate i (the last one if there are fewer)
+int forced_candidate() {
+  const char *c = std::getenv("SGW_AUTOTUNE_FORCE");
+  return c ? std::atoi(c) : -1;
+}
+
+}  // namespace
+
+int autotune_choice(const std::string &key) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_choice.find(key);
+  return it == g_choice.end() ? -1 : it->second;
+}
+
+int autotune(const std::string &key, int ncand, hipStream_t st,
+             const std::function<void(int)> &launch) {
+  const int forced = forced_candidate();
+  if (forced >= 0) {
+    const int c = forced < ncand ? forced : ncand - 1;
+    launch(c);
+    return c;
+  }
+  if (ncand <= 1 || !tuning_enabled()) {
+    launch(0);
+    return 0;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_choice.find(key);
+    if (it != g_choice.end()) {
+      launch(it->second);
+      return it->second;
+    }
+  }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  SGW_HIP(hipStreamIsCapturing(st, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    launch(0);
+    return 0;
+  }
+  // each candidate twice (the first launch of a kernel also loads its code);
+  // the second timing counts
+  hipEvent_t e0, e1;
+  SGW_HIP(hipEventCreate(&e0));
+  SGW_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = 0.f;
+  try {
+    for (int i = 0; i < ncand; ++i) {
+      launch(i);
+      SGW_HIP(hipEventRecord(e0, st));
+      launch(i);
+      SGW_HIP(hipEventRecord(e1, st));
+      SGW_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      SGW_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (i == 0 || ms < best_ms) {
+        best = i;
+        best_ms = ms;
+      }
+    }
+  } catch (...) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    throw;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_choice[key] = best;
+  return best;
+}
+
+}  // namespace sgw
+
+extern "C" {
+
+// The candidate the autotuner chose for `key`, -1 if it has not chosen one.
+int sgw_autotune_choice(const char *key) {
+  return sgw::autotune_choice(key ? key : "");
+}
+
+}  // extern "C"

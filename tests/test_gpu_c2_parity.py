@@ -138,3 +138,32 @@ def test_streamed_solve_and_residual_check_against_oracle():
             r = np.linalg.norm(rhs[i, :n] - ops[g].apply(x[i, k, :n], sig[i, k])) / bn
             assert r <= 1e-8, (i, k, r)
             assert abs(res[i, k] - r) <= 1e-3 * r + 1e-13, (i, k, res[i, k], r)
+
+
+def test_autotuned_shapes_agree_bitwise(monkeypatch):
+    """Every launch shape the autotuner may pick for the 48^3 plane
+    transforms and the assembly (SGW_AUTOTUNE_FORCE=i) gives the default
+    shape's H-apply bit for bit, and the tuned run records a choice."""
+    w = _cell(8)
+    op = operators.PwOperator.from_workload(w)
+    rng = np.random.Generator(np.random.PCG64(5))
+    nv = 21
+    grp = (np.arange(nv) % 2).astype(np.int32)
+    x = np.zeros((nv, w.ld), dtype=np.complex128)
+    for v in range(nv):
+        n = w.npw[grp[v]]
+        x[v, :n] = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    sig = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
+    out = []
+    for c in ("0", "1", "2"):
+        monkeypatch.setenv("SGW_AUTOTUNE_FORCE", c)
+        out.append(op.apply(x, sig, group=grp))
+    monkeypatch.delenv("SGW_AUTOTUNE_FORCE")
+    tuned = op.apply(x, sig, group=grp)
+    for o in out[1:] + [tuned]:
+        assert np.array_equal(o, out[0])
+    L = _lib.lib()
+    assert L.sgw_autotune_choice(b"plane48/M8/b8/n32") in (0, 1, 2)
+    assert L.sgw_autotune_choice(b"assemble/n32/g2/p36") in (0, 1, 2)
+    ref = orc.PwOp(w, 0).apply(x[0, :w.npw[0]], sig[0])
+    assert np.linalg.norm(tuned[0, :w.npw[0]] - ref) <= 1e-12 * np.linalg.norm(ref)
