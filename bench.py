@@ -38,6 +38,8 @@ sys.path.insert(0, ROOT)
 
 RES_TARGET = 1e-8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "happly_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "happly_fp64_valu.json")
+FP64_PEAK = 78.6  # TFLOP/s, MI355X FP64 (vector and matrix)
 
 
 def parse():
@@ -232,11 +234,9 @@ def run_response(args):
     ms, cnt = ctypes.c_double(), ctypes.c_int64()
     _lib.check(L.sgw_kernel_timing_query(b"happly", ctypes.byref(ms), ctypes.byref(cnt)))
     _lib.check(L.sgw_kernel_timing(0))
-    nbox = int(np.prod(w.box))
-    npw = float(w.npw[0])
-    bh = algorithmic_bytes_per_happly(nbox, npw, w.nproj, w.nbnd)
     vec = stats.get("vector_happlies", 0)
-    achieved = bh * vec / (ms.value / 1e3) / 1e9 if ms.value > 0 else 0.0
+    b_launch, _bv, _bs = pruned_bytes_per_launch(w, vec / max(cnt.value, 1))
+    achieved = b_launch * cnt.value / (ms.value / 1e3) / 1e9 if ms.value > 0 else 0.0
     out = {
         "metric": "dielectric-matrix columns/s (direct solve_linter + coulomb, 48^3 FFT, "
                   "16 bands, 31 shifts, 1e-8)",
@@ -252,6 +252,7 @@ def run_response(args):
         "roofline": {"kernel": "H-apply (pw_becp_zinv+pw_reduce_plane+pw_zfwd+pw_assemble)",
                      "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0,
                      "unit": "GB/s", "frac": round(achieved / 8000.0, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": round(b_launch),
                      "vector_happlies": vec, "launches": cnt.value,
                      "avg_launch_ms": round(ms.value / max(cnt.value, 1), 4)},
     }
@@ -360,15 +361,24 @@ def run_sigma_real(args):
     print(json.dumps(out))
 
 
-def algorithmic_bytes_per_happly(nbox, npw, nproj, nvec_per_group, sc=16, sr=8):
-    """B_H per H-apply per vector: a 3-pass fused FFT pipeline touching the box
-    4 times (complex) + V(r) once + psi in/out + projectors amortised over the
-    vectors sharing them."""
-    return 4 * nbox * sc + nbox * sr + 2 * npw * sc + 2 * nproj * npw * sc / nvec_per_group
+def sphere_columns(sphere):
+    """Number of (m1, m2) z-columns the plane-wave sphere occupies."""
+    return len({(int(m[0]), int(m[1])) for m in sphere.mill})
 
 
-def flops_per_happly(nbox, npw, nproj):
-    return 2 * 5 * nbox * math.log2(nbox) + 6 * nbox + 16 * nproj * npw
+def pruned_bytes_per_launch(w, nvec, sc=16, sr=8):
+    """Algorithmic HBM bytes of one sphere-pruned H-apply launch over nvec
+    vectors (the pipeline this library runs): per vector, the plane waves in
+    and out (2 N_pw) and the z-stick intermediates between the z and the
+    plane passes, written once and read once in each direction
+    (4 n3 N_col, N_col the sphere's occupied columns); per launch, V(r) once
+    and every group's projectors once.  N_pw and N_col are the means over
+    the batch's systems."""
+    npw_sys = w.npw[w.group].astype(float)
+    ncol = np.array([sphere_columns(s) for s in w.spheres], dtype=float)
+    per_vec = 2 * npw_sys.mean() * sc + 4 * w.box[2] * ncol[w.group].mean() * sc
+    shared = int(np.prod(w.box)) * sr + w.nproj * float(w.npw.sum()) * sc
+    return nvec * per_vec + shared, per_vec, shared
 
 
 def _cpu_worker(job):
@@ -603,31 +613,37 @@ def main():
     solves_per_s = good * (1 if args.stream else args.steps) / elapsed
 
     # roofline of the H-apply (one event pair around its kernels on the solver stream)
-    nbox = int(np.prod(w.box))
-    npw_sys = w.npw[w.group]
-    mean_npw = float(npw_sys.mean())
-    bh = algorithmic_bytes_per_happly(nbox, mean_npw, w.nproj, args.nbnd)
-    fh = flops_per_happly(nbox, mean_npw, w.nproj)
     t_h = happly[0] / 1e3
     launches = happly[1]
-    achieved = bh * total_vec_happly / t_h / 1e9 if t_h > 0 else 0.0
+    vpl = total_vec_happly / max(launches, 1)
+    b_launch, b_vec, b_shared = pruned_bytes_per_launch(w, vpl)
+    achieved = b_launch / (t_h / max(launches, 1)) / 1e9 if t_h > 0 else 0.0
 
-    traffic, traffic_note = None, None
+    traffic, traffic_note, fp64 = None, None, None
+    key = [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold, args.shift_test, relax,
+           args.stream, args.steps if args.stream else 1, slots if args.stream else nsys]
     if os.path.exists(TRAFFIC_FILE):
         # HBM bytes per H-apply from two rocprofv3 --pmc runs of this command
         # (FETCH_SIZE, WRITE_SIZE; each scaled by a plain copy of known size)
         with open(TRAFFIC_FILE) as f:
             prof = json.load(f)
-        if prof.get("workload") == [args.nq, args.nbnd, args.nshift, args.lmax, args.threshold,
-                                    args.shift_test, relax, args.stream,
-                                    args.steps if args.stream else 1,
-                                    slots if args.stream else nsys]:
+        if prof.get("workload") == key:
             traffic = prof["traffic_bytes_per_happly"]
-            traffic_note = {"source": "profiles/happly_traffic.json",
-                            "algorithmic_bytes_per_launch": round(bh * total_vec_happly
-                                                                  / max(launches, 1)),
-                            "traffic_over_algorithmic": round(
-                                traffic / (bh * total_vec_happly / max(launches, 1)), 3)}
+            traffic_note = {"source": os.path.relpath(TRAFFIC_FILE, ROOT),
+                            "traffic_over_algorithmic": round(traffic / b_launch, 3),
+                            "measured_hbm_GBps": (round(traffic / (t_h / max(launches, 1)) / 1e9, 1)
+                                                  if t_h else None)}
+    if os.path.exists(VALU_FILE):
+        # FP64 VALU instructions per H-apply from rocprofv3 --pmc runs of this
+        # command (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the four kernels)
+        with open(VALU_FILE) as f:
+            prof = json.load(f)
+        if prof.get("workload") == key:
+            flops = prof["fp64_flops_per_happly"]
+            rate = flops / (t_h / max(launches, 1)) / 1e12 if t_h else 0.0
+            fp64 = {"source": os.path.relpath(VALU_FILE, ROOT),
+                    "flops_per_launch": flops, "achieved_TFLOPs": round(rate, 2),
+                    "peak_TFLOPs": FP64_PEAK, "frac": round(rate / FP64_PEAK, 4)}
 
     out = {
         "metric": "Sternheimer solves/s at 1e-8 residual (48^3 FFT, 32 shifts); % HBM roofline",
@@ -665,20 +681,16 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / 8000.0, 4),
             "traffic": traffic,
+            "algorithmic_bytes_per_launch": round(b_launch),
+            "algorithmic_bytes_per_vector": round(b_vec),
+            "algorithmic_bytes_shared_per_launch": round(b_shared),
             "traffic_detail": traffic_note,
-            "algorithmic_bytes_per_vector": round(bh),
+            "fp64_valu": fp64,
             "vector_happlies": total_vec_happly,
             "launches": launches,
-            "vectors_per_launch": round(total_vec_happly / max(launches, 1), 1),
+            "vectors_per_launch": round(vpl, 1),
             "avg_launch_ms": round(t_h * 1e3 / max(launches, 1), 4),
             "happly_time_share": round(t_h / elapsed, 4),
-            "fp64_tflops_equiv": round(fh * total_vec_happly / t_h / 1e12, 2) if t_h else 0.0,
-            # achieved is B_H, the bytes of a three-pass full-box pipeline; the
-            # sphere-pruned pipeline moves a fraction of them (traffic), so frac
-            # can pass 1.  measured_hbm_GBps is the counted traffic per launch
-            # over the launch time.
-            "measured_hbm_GBps": (round(traffic / (t_h / max(launches, 1)) / 1e9, 1)
-                                  if traffic and t_h else None),
         },
     }
     if breakdown is not None:

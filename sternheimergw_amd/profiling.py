@@ -10,6 +10,12 @@
         the plane-wave H-apply as one unit (all k_pw_* kernels of one
         application): average duration per application from the kernel trace
         and calibrated HBM bytes read and written per application, as JSON
+    python -m sternheimergw_amd.profiling sq <run_results.db> [more.db ...] [--skip N]
+        per-kernel average of every counter of the databases (SQ_* and the
+        like, one row per kernel and counter), then, as JSON, the FP64 VALU
+        flops of one H-apply: 64 lanes x (ADD + MUL + TRANS + 2 FMA) FP64
+        instructions summed over the k_pw_* kernels (full wavefronts
+        assumed, so partial execution masks count as full)
 """
 from __future__ import annotations
 
@@ -120,10 +126,49 @@ def happly(trace_db: str, fetch_db: str, write_db: str, copy_bytes: int,
     return out
 
 
+def sq(dbs, skip_last: int = 0):
+    """Per-kernel averages of every counter in `dbs` and the H-apply's FP64
+    VALU flops (see the module text)."""
+    agg = collections.defaultdict(list)
+    for db in dbs:
+        con = sqlite3.connect(db)
+        rows = con.execute("select kernel_name, counter_name, value from counters_collection "
+                           "order by dispatch_id").fetchall()
+        for name, cname, val in rows:
+            agg[(_short(name), cname)].append(val)
+    if skip_last:
+        agg = {k: (v[:-skip_last] if "k_pw_" in k[0] and len(v) > skip_last else v)
+               for k, v in agg.items()}
+    lines = ["| kernel | counter | dispatches | avg per dispatch |", "|---|---|---|---|"]
+    for (name, cname), vals in sorted(agg.items(), key=lambda kv: (kv[0][0], kv[0][1])):
+        lines.append(f"| `{name}` | {cname} | {len(vals)} | {sum(vals) / len(vals):.4g} |")
+    weight = {"SQ_INSTS_VALU_ADD_F64": 1, "SQ_INSTS_VALU_MUL_F64": 1,
+              "SQ_INSTS_VALU_TRANS_F64": 1, "SQ_INSTS_VALU_FMA_F64": 2}
+    per_kernel = collections.defaultdict(float)
+    for (name, cname), vals in agg.items():
+        if "k_pw_" in name and cname in weight:
+            per_kernel[name] += 64.0 * weight[cname] * sum(vals) / len(vals)
+    out = {"fp64_flops_per_kernel": {k: round(v) for k, v in per_kernel.items()},
+           "fp64_flops_per_happly": round(sum(per_kernel.values()))}
+    return "\n".join(lines), out
+
+
 def main(argv):
-    if len(argv) < 3 or argv[1] not in ("trace", "pmc", "happly"):
+    if len(argv) < 3 or argv[1] not in ("trace", "pmc", "happly", "sq"):
         print(__doc__)
         return 2
+    if argv[1] == "sq":
+        import json
+        args = argv[2:]
+        skip = 0
+        if "--skip" in args:
+            i = args.index("--skip")
+            skip = int(args[i + 1])
+            args = args[:i] + args[i + 2:]
+        table, out = sq(args, skip)
+        print(table)
+        print(json.dumps(out, indent=1))
+        return 0
     if argv[1] == "happly":
         import json
         skip = int(argv[6]) if len(argv) > 6 else 0
