@@ -66,6 +66,11 @@ def parse():
                     help="mixed: FP32 local-potential FFTs once a system's residual estimate "
                          "is below --relax-threshold (vectors, recurrences and test in FP64)")
     ap.add_argument("--relax-threshold", type=float, default=1e-4)
+    ap.add_argument("--inner-threshold", type=float, default=1e-5,
+                    help="mixed workload: threshold of the FP32-FFT multishift stage")
+    ap.add_argument("--mixed-cell", type=int, choices=(24, 48), default=24,
+                    help="mixed workload: the 24^3 cell with chi0 columns at N_G = 500, or the "
+                         "48^3 batch of the headline (dpsi errors only)")
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the CPU oracle on a sample of the batch (0 to skip)")
     ap.add_argument("--kernel-timing", type=int, default=1,
@@ -75,12 +80,14 @@ def parse():
                     help="after the timed steps, one untimed step with per-kernel events")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (MiB) for counter calibration")
-    ap.add_argument("--workload", choices=("solve", "sigma", "response", "sigma_real"),
+    ap.add_argument("--workload", choices=("solve", "sigma", "response", "sigma_real", "mixed"),
                     default="solve",
                     help="solve: the batched Sternheimer solve (headline); sigma: the Sigma_c "
                          "frequency convolution GEMM (N_G = 1500, 64 x 64 frequencies); "
                          "response: one dielectric column (direct solve_linter, 48^3); "
-                         "sigma_real: the reference-form Sigma_c products (24^3, N_G 1500)")
+                         "sigma_real: the reference-form Sigma_c products (24^3, N_G 1500); "
+                         "mixed: FP32 FFTs with FP64 refinement against FP64 (24^3, chi0 "
+                         "columns at N_G 500)")
     ap.add_argument("--sigma-precision", choices=("fp64", "fp32s"), default="fp64",
                     help="fp32s: FP32 MFMA products, per-frequency FP32 sums, FP64 frequency sum")
     ap.add_argument("--sigma-batch", type=int, default=8,
@@ -361,6 +368,154 @@ def run_sigma_real(args):
     print(json.dumps(out))
 
 
+def run_mixed(args):
+    """The mixed-precision configuration: the 24^3 cell of the plain FP64
+    configuration (6 Ha, 4 bands, one q, 8 shifts i linspace(0, 2 Ha)) with the
+    plane-wave perturbations e^{i(q+G)r} of the 500 shortest G, i.e. the
+    columns of chi0(G, G'; i omega) at N_G = 500: 2000 systems x 8 shifts.
+    A step solves them all (right-hand sides unit norm, threshold 1e-8 on
+    every shift) three ways: FP64 (the reference's arithmetic), FP32 FFTs with
+    FP64 refinement (sgw_sternheimer_solve_refined), and FP32 FFTs relaxed in
+    once a residual estimate is below 1e-4 (vectors FP64, no refinement).
+    Reported: solves/s of each, the ratio to FP64, and max/rms errors of
+    dpsi and of chi0(G, G'; i omega) = sum_v b_v^G+ dpsi_v^G' against the FP64
+    solutions, which are checked against the oracle on a sample."""
+    import torch
+
+    from oracle import oracle as orc
+    from sternheimergw_amd import _lib, cell, mixed, operators
+    from sternheimergw_amd.batch import BatchSolver
+
+    dev = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    ng = 500
+    big = args.mixed_cell == 48
+    if big:  # the 48^3 batch of the headline (no perturbation columns: dpsi errors only)
+        w = cell.workload_c2(nq=args.nq, nbnd=args.nbnd, nshift=args.nshift,
+                             normalize_rhs=False)
+    else:
+        w = cell.workload_c1(keep_dense=False, pert_mill=cell.shortest_millers(ng),
+                             normalize_rhs=False)
+    bnorm = np.linalg.norm(w.rhs, axis=1)
+    rhs = w.rhs / bnorm[:, None]
+    op = operators.PwOperator.from_workload(w)
+    nsys, nshift, ld = w.nsys, w.nshift, w.ld
+    d_rhs = torch.from_numpy(rhs).to(dev)
+    d_sig = torch.from_numpy(np.ascontiguousarray(w.sigma)).to(dev)
+    d_g = torch.from_numpy(w.group).to(dev)
+    thr = args.threshold
+    setup_s = time.perf_counter() - t_setup
+
+    def run(kind):
+        x = torch.zeros((nsys, nshift, ld), dtype=torch.complex128, device=dev)
+        extra = {}
+        if kind == "fp64":
+            sol = BatchSolver(op, nsys, nshift, ld, lmax=args.lmax, threshold=thr,
+                              shift_test=True)
+            f = lambda: sol.solve(d_g, d_rhs, d_sig, w.sigma, x)
+        elif kind == "relaxed":
+            sol = BatchSolver(op, nsys, nshift, ld, lmax=args.lmax, threshold=thr,
+                              shift_test=True, relax_threshold=args.relax_threshold)
+            f = lambda: sol.solve(d_g, d_rhs, d_sig, w.sigma, x)
+        else:
+            sol = BatchSolver(op, nsys, nshift, ld, lmax=args.lmax, threshold=thr,
+                              shift_test=True)
+            f = lambda: extra.update(mixed.solve_refined(sol, d_g, d_rhs, d_sig, w.sigma, x,
+                                                         inner_threshold=args.inner_threshold,
+                                                         max_refine=3)[1])
+        for _ in range(args.warmup):
+            f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / args.steps
+        d_res = torch.zeros((nsys, nshift), dtype=torch.float64, device=dev)
+        _lib.check(_lib.lib().sgw_residual_norms_device(
+            op.handle, nsys, nshift, ld, ctypes.c_void_p(d_g.data_ptr()),
+            ctypes.c_void_p(d_rhs.data_ptr()), ctypes.c_void_p(d_sig.data_ptr()),
+            ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(d_res.data_ptr()), None))
+        res = d_res.cpu().numpy()
+        print(f"{kind}: {el * 1e3:.1f} ms per step, max residual {res.max():.2e}", flush=True)
+        return x.cpu().numpy(), el, res, extra
+
+    out64, t64, res64, _ = run("fp64")
+    outr, tr, resr, stats = run("refined")
+    outx, tx, resx, _ = run("relaxed")
+    nsolves = nsys * nshift
+
+    def errors(x):
+        d = np.linalg.norm(x - out64, axis=2) / np.linalg.norm(out64, axis=2)
+        if big:
+            return {"dpsi_max_rel": float(d.max()), "dpsi_rms_rel": float(np.sqrt((d ** 2).mean()))}
+        # chi0[k][G][G'] = sum_v b_v^G+ dpsi_v^G'(k) with the unnormalised b
+        xb = x * bnorm[:, None, None]
+        x6 = out64 * bnorm[:, None, None]
+        nb, npert = w.nbnd, w.npert
+        b = w.rhs.reshape(1, nb, npert, ld)[0]
+        chi = np.einsum("vgl,vhkl->kgh", b.conj(), xb.reshape(nb, npert, nshift, ld))
+        chi6 = np.einsum("vgl,vhkl->kgh", b.conj(), x6.reshape(nb, npert, nshift, ld))
+        dc = np.abs(chi - chi6)
+        return {"dpsi_max_rel": float(d.max()), "dpsi_rms_rel": float(np.sqrt((d ** 2).mean())),
+                "chi0_max_abs_over_max": float(dc.max() / np.abs(chi6).max()),
+                "chi0_rms_rel": float(np.sqrt((dc ** 2).sum() / (np.abs(chi6) ** 2).sum()))}
+
+    # the FP64 solutions against the oracle on a sample of systems
+    rng = np.random.Generator(np.random.PCG64(3))
+    sample = rng.choice(nsys, size=8, replace=False)
+    worst = 0.0
+    tc = time.perf_counter()
+    if big:
+        sample = sample[:2]
+    for s_ in sample:
+        o = orc.PwOp(w, int(w.group[s_]))
+        n = int(w.npw[w.group[s_]])
+        xo, eo, _ = orc.bicgstab(o, rhs[s_, :n], w.sigma[s_], lmax=args.lmax, threshold=1e-12)
+        rel = np.linalg.norm(out64[s_, :, :n] - xo, axis=1) / np.linalg.norm(xo, axis=1)
+        worst = max(worst, float(rel.max()))
+    tc = time.perf_counter() - tc
+    er, ex = errors(outr), errors(outx)
+    good = lambda r: int((r <= thr).sum())
+    out = {
+        "metric": ("Mixed-precision Sternheimer solves/s at 1e-8 residual (48^3 FFT, 32 shifts): "
+                   "FP32 FFT + FP64 refinement" if big else
+                   "Mixed-precision Sternheimer solves/s at 1e-8 residual (24^3 FFT, 8 shifts, "
+                   "chi0 columns at N_G=500): FP32 FFT + FP64 refinement"),
+        "value": round(good(resr) / tr, 2),
+        "unit": "solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(tr * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64 local-potential FFTs, c128 vectors, scalars, projectors and residuals",
+        "data": "synthetic (seeded fcc cell, V_loc, beta projectors; FP64 diagonalisation)",
+        "config": {"workload": ("batched-mixed-48^3: 16 bands x 16 q x 32 shifts, 25 Ha, "
+                                "BiCGStab(4), lock-step, refinement to 1e-8 per shift" if big else
+                                "mixed-24^3: 4 bands x 500 G perturbations x 8 shifts, 6 Ha, "
+                                "BiCGStab(4), refinement to 1e-8 per shift"),
+                   "nsys": nsys, "nshift": nshift, "threshold": thr,
+                   "inner_threshold": args.inner_threshold, "setup_s": round(setup_s, 2),
+                   "solves_per_step": nsolves},
+        "fp64": {"solves_per_s": round(good(res64) / t64, 2), "ms_per_step": round(t64 * 1e3, 3),
+                 "solves_meeting_target": good(res64), "max_residual": float(res64.max()),
+                 "max_rel_diff_vs_oracle_sample": worst,
+                 "oracle_sample": f"{len(sample)} systems, oracle at 1e-12, {tc:.1f} s"},
+        "refined": {"solves_per_s": round(good(resr) / tr, 2), "ratio_to_fp64": round(t64 / tr, 3),
+                    "solves_meeting_target": good(resr), "max_residual": float(resr.max()),
+                    "errors_vs_fp64": er, "stats": stats},
+        "relaxed": {"solves_per_s": round(good(resx) / tx, 2), "ratio_to_fp64": round(t64 / tx, 3),
+                    "relax_threshold": args.relax_threshold,
+                    "solves_meeting_target": good(resx), "max_residual": float(resx.max()),
+                    "errors_vs_fp64": ex},
+    }
+    if args.cpu_baseline:
+        out["cpu_baseline"] = {"value": round(len(sample) * nshift / tc, 3), "unit": "solves/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{len(sample)} systems x {nshift} shifts with the oracle "
+                                         f"(FP64, threshold 1e-12) on one core, {tc:.1f} s"}
+    print(json.dumps(out))
+
+
 def sphere_columns(sphere):
     """Number of (m1, m2) z-columns the plane-wave sphere occupies."""
     return len({(int(m[0]), int(m[1])) for m in sphere.mill})
@@ -456,6 +611,9 @@ def main():
         return
     if args.workload == "sigma_real":
         run_sigma_real(args)
+        return
+    if args.workload == "mixed":
+        run_mixed(args)
         return
     import torch
 

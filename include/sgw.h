@@ -240,6 +240,31 @@ int sgw_sternheimer_solve_batched(sgw_operator *op, const sgw_batch_config *cfg,
                                   const sgw_complex *sigma, sgw_complex *x, int32_t *ierr,
                                   int32_t *iters);
 
+/* Mixed-precision batched solve with FP64 iterative refinement.  No
+ * counterpart in the reference (FP64 throughout); the FP64 steps are those of
+ * bicgstab.f90:154-275.  On `solver` (plane-wave operator) it
+ *   1. solves the multishift systems with the local-potential FFTs in FP32
+ *      down to `inner_threshold` (the solver's seed/shift test);
+ *   2. forms every shift's residual r = b - (A + sigma) x in FP64;
+ *   3. for each (system, shift) above the solver's threshold (relative to
+ *      ||b||), solves the single-shift correction (A + sigma) d = r with FP32
+ *      FFTs, streamed through `correction_slots` slots, and adds d to x;
+ *      then back to 2, at most max_refine times.
+ * The shifts need step 3 because FP32 products break the collinearity of the
+ * shifted residuals that the multishift recurrences rely on.  Vectors,
+ * recurrence scalars, the projector terms (including alpha_pv P_v) and the
+ * residuals are FP64.  Arrays as in sgw_batch_solver_run.  h_ierr[nsys]
+ * (nullable): 0 when every shift's FP64 residual meets the threshold;
+ * h_stats[6] (nullable): FP32 H-applies of step 1 and of step 3, FP64
+ * residual applications, refinement rounds, step-1 iterations, step-3
+ * iterations; h_res[nsys*nshift] (nullable): final relative residuals. */
+int sgw_sternheimer_solve_refined(sgw_batch_solver *solver, const int32_t *d_group,
+                                  const sgw_complex *d_rhs, const sgw_complex *d_sigma,
+                                  const sgw_complex *h_sigma, sgw_complex *d_x,
+                                  double inner_threshold, int32_t max_refine,
+                                  int32_t correction_slots, int32_t *h_ierr, int64_t *h_stats,
+                                  double *h_res, void *stream);
+
 /* Adjoint of the batched solve (no counterpart in the reference, which has no
  * derivatives).  For the objective
  *   L = sum_{s,k} weight[s][k] Re <c[s] | x[s][k]>,

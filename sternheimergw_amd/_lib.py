@@ -152,6 +152,9 @@ SIGNATURES = [
     ("sgw_kernel_timing_query", c_int32, [ctypes.c_char_p, ctypes.POINTER(c_double),
                                           ctypes.POINTER(c_int64)]),
     ("sgw_autotune_choice", c_int32, [ctypes.c_char_p]),
+    ("sgw_sternheimer_solve_refined", c_int32, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                c_void_p, c_double, c_int32, c_int32, c_void_p,
+                                                c_void_p, c_void_p, c_void_p]),
     ("sgw_last_error", ctypes.c_char_p, []),
     ("sgw_version", ctypes.c_char_p, []),
 ]

@@ -36,7 +36,8 @@ class BatchBicgstab {
   void solve_stream(Operator &op, hipStream_t st, int nq, const double2 *d_rhs, int64_t rhs_ld,
                     const double2 *d_sigma, const double2 *h_sigma, const int32_t *h_group,
                     double threshold, int max_iter, double2 *d_x, int64_t x_ld,
-                    int32_t *h_ierr, int32_t *h_iters, int shift_test);
+                    int32_t *h_ierr, int32_t *h_iters, int shift_test,
+                    double relax_thr = 0.0);
 
  private:
   struct Impl;

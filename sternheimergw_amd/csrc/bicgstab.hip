@@ -600,7 +600,8 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
                                  Scal sc, const double2 *__restrict__ sigma_all, int nshift_all,
                                  double2 *__restrict__ sigma_seed, int32_t *__restrict__ active,
                                  int32_t *__restrict__ iters, int32_t *__restrict__ state,
-                                 int32_t *__restrict__ fresh, int32_t *__restrict__ count) {
+                                 int32_t *__restrict__ fresh, int32_t *__restrict__ count,
+                                 int32_t *__restrict__ relax) {
   // one thread per (pair, shifted system); the k = 0 thread also does the seed
   const int nshw = nsh > 0 ? nsh : 1;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -616,6 +617,7 @@ __global__ void k_refill_scalars(int npairs, const int32_t *__restrict__ pairs, 
   state[s] = 0;
   fresh[s] = 1;
   count[s] = 0;
+  if (relax) relax[s] = 0;
 }
 
 // Compaction of running systems into the leading slots: pairs[2p] = source
@@ -1241,7 +1243,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t caller_st, int nq, co
                                  int64_t rhs_ld, const double2 *d_sigma, const double2 *h_sigma,
                                  const int32_t *h_group, double threshold, int max_iter,
                                  double2 *d_x, int64_t x_ld, int32_t *h_ierr, int32_t *h_iters,
-                                 int shift_test) {
+                                 int shift_test, double relax_thr) {
   Impl &m = *impl_;
   const int S = m.nsys;
   SGW_REQUIRE(!op.host_side(), SGW_E_ARG, "bicgstab stream: device-resident operators only");
@@ -1260,7 +1262,23 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t caller_st, int nq, co
   SGW_REQUIRE(nq >= 0 && h_group && rhs_ld >= m.ld, SGW_E_ARG,
               "bicgstab stream: need nq >= 0, host group indices and rhs_ld >= ld");
   m.shift_test = shift_test;
-  m.relax_thr = 0.0;
+  // relaxed FP32 local-potential FFTs per slot (plane-wave operators only);
+  // a refilled slot starts in FP64
+  auto *pw = dynamic_cast<PwOperator *>(&op);
+  m.relax_thr = (pw != nullptr && relax_thr > 0) ? relax_thr : 0.0;
+  struct RelaxGuard {
+    PwOperator *pw;
+    int mode;
+    ~RelaxGuard() {
+      if (pw) pw->set_fft_precision(mode, nullptr);
+    }
+  } guard{nullptr, 0};
+  if (m.relax_thr > 0) {
+    if (m.relax.n < (size_t)S) m.relax.alloc(S);
+    m.relax.zero(st);
+    guard = RelaxGuard{pw, pw->fft_precision()};
+    pw->set_fft_precision(2, m.relax.p);
+  }
   // slots per group, in proportion to the queued systems of each group
   const int ng = op.ngroups();
   std::vector<std::vector<int>> queue(ng);
@@ -1343,7 +1361,7 @@ void BatchBicgstab::solve_stream(Operator &op, hipStream_t caller_st, int nq, co
                                                                            rhs_ld);
     k_refill_scalars<<<grid1((int)pr.size() * std::max(m.nsh, 1), 64), 64, 0, st>>>(
         (int)pr.size(), dp, m.nsh, m.l, m.sc, d_sigma, m.nshift, m.sigma_seed.p, m.active.p,
-        m.iters.p, m.state.p, m.fresh.p, m.count.p);
+        m.iters.p, m.state.p, m.fresh.p, m.count.p, m.relax_thr > 0 ? m.relax.p : nullptr);
   };
   // first fill
   std::vector<std::pair<int, int>> pr;

@@ -379,7 +379,8 @@ int sgw_batch_solver_run_stream(sgw_batch_solver *solver, int32_t nq, const int3
     std::vector<int32_t> ierr(std::max(nq, 1)), it(std::max(nq, 1));
     solver->engine->solve_stream(*solver->op->op, (hipStream_t)stream, nq, D2(d_rhs), c.ld,
                                  D2(d_sigma), D2(h_sigma), h_group, c.threshold, c.max_iter,
-                                 D2(d_x), c.ld, ierr.data(), it.data(), c.shift_test);
+                                 D2(d_x), c.ld, ierr.data(), it.data(), c.shift_test,
+                                 c.relax_threshold);
     int worst = 0;
     for (int q = 0; q < nq; ++q) worst = std::max(worst, ierr[q]);
     if (h_ierr && nq > 0) std::memcpy(h_ierr, ierr.data(), sizeof(int32_t) * nq);

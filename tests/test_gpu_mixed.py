@@ -102,3 +102,63 @@ def test_mixed_batch_48cubed():
         _lib.c_void_p(d_x.data_ptr()), _lib.c_void_p(d_res.data_ptr()), None))
     torch.cuda.synchronize()
     assert d_res.max().item() <= TARGET
+
+
+def test_refined_solve_meets_target_and_oracle(c1):
+    """FP32-FFT multishift solve to 1e-5, then FP64 residuals and FP32-FFT
+    single-shift corrections (sgw_sternheimer_solve_refined): every shift's
+    true residual (recomputed by the oracle's FP64 operator) meets 1e-9, and
+    every solution is within 1e-8 relative of the oracle's FP64 solve."""
+    from sternheimergw_amd import mixed
+    w = c1
+    op = operators.PwOperator.from_workload(w)
+    sol = batch.BatchSolver(op, w.nsys, w.nshift, w.ld, lmax=4, threshold=1e-9, shift_test=True)
+    d_rhs = torch.from_numpy(w.rhs).to(DEV)
+    d_sig = torch.from_numpy(np.ascontiguousarray(w.sigma)).to(DEV)
+    d_g = torch.from_numpy(w.group).to(DEV)
+    d_x = torch.zeros((w.nsys, w.nshift, w.ld), dtype=torch.complex128, device=DEV)
+    print("refined solve 24^3", flush=True)
+    ierr, stats, res = mixed.solve_refined(sol, d_g, d_rhs, d_sig, w.sigma, d_x,
+                                           inner_threshold=1e-5, max_refine=3)
+    assert np.all(ierr == 0), (ierr, stats)
+    assert stats["refinements"] >= 1 and stats["fp32_happly_corrections"] > 0
+    x = d_x.cpu().numpy()
+    for s in range(w.nsys):
+        g = int(w.group[s])
+        n = int(w.npw[g])
+        o = orc.PwOp(w, g)
+        xo, eo, _ = orc.bicgstab(o, w.rhs[s, :n], w.sigma[s], lmax=4, threshold=1e-12)
+        assert eo == 0
+        bn = np.linalg.norm(w.rhs[s, :n])
+        for k in range(w.nshift):
+            r = np.linalg.norm(w.rhs[s, :n] - o.apply(x[s, k, :n], w.sigma[s, k])) / bn
+            assert r <= 1e-9, (s, k, r)
+            assert abs(res[s, k] - r) <= 1e-3 * r + 1e-14
+            assert np.linalg.norm(x[s, k, :n] - xo[k]) <= 1e-8 * np.linalg.norm(xo[k])
+
+
+def test_streamed_relaxed_mixed_48cubed():
+    """The relaxed form (FP32 FFTs once a slot's residual estimate is below
+    1e-4) through the streamed solver: a refilled slot starts again in FP64,
+    and every solve's FP64 residual meets 1e-8."""
+    w = cell.workload_c2(nq=2, nbnd=4)
+    op = operators.PwOperator.from_workload(w)
+    order = np.array([0, 4, 1, 5, 2, 6, 3, 7, 0, 7])
+    rhs, sig, grp = w.rhs[order], np.ascontiguousarray(w.sigma[order]), w.group[order]
+    nq, ns, ld = len(order), w.nshift, w.ld
+    sol = batch.BatchSolver(op, 4, ns, ld, lmax=4, threshold=1e-8, shift_test=True,
+                            relax_threshold=1e-4)
+    d_x = torch.zeros((nq, ns, ld), dtype=torch.complex128, device=DEV)
+    d_rhs = torch.from_numpy(rhs).to(DEV)
+    d_sig = torch.from_numpy(sig).to(DEV)
+    d_g = torch.from_numpy(grp).to(DEV)
+    print("streamed relaxed mixed 48^3: 10 systems through 4 slots", flush=True)
+    ierr, _ = sol.solve_stream(grp, d_rhs, d_sig, sig, d_x)
+    assert np.all(ierr == 0)
+    d_res = torch.zeros((nq, ns), dtype=torch.float64, device=DEV)
+    _lib.check(_lib.lib().sgw_residual_norms_device(
+        op.handle, nq, ns, ld, _lib.c_void_p(d_g.data_ptr()), _lib.c_void_p(d_rhs.data_ptr()),
+        _lib.c_void_p(d_sig.data_ptr()), _lib.c_void_p(d_x.data_ptr()),
+        _lib.c_void_p(d_res.data_ptr()), None))
+    torch.cuda.synchronize()
+    assert d_res.max().item() <= TARGET
