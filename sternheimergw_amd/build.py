@@ -96,16 +96,20 @@ def build_fortran(verbose: bool = False):
                                       if os.path.exists("/opt/rocm/bin/amdflang") else None)
     if fc is None:
         return None
-    srcs = ["sgw_kinds.f90", "select_solver_module.f90", "sgw_fixture_test.f90"]
     mod = os.path.join(OBJDIR, "fmod")
     os.makedirs(mod, exist_ok=True)
-    exe = os.path.join(FORTRAN, "sgw_fixture_test")
-    cmd = [fc, "-O2", "-J", mod, *[os.path.join(FORTRAN, s) for s in srcs],
-           "-L" + HERE, "-lsgw_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
-    if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
-    return exe
+    modules = ["sgw_kinds.f90", "select_solver_module.f90", "sgw_callers.f90"]
+    exes = []
+    for driver in ("sgw_fixture_test", "sgw_callers_test"):
+        exe = os.path.join(FORTRAN, driver)
+        cmd = [fc, "-O2", "-J", mod, *[os.path.join(FORTRAN, s) for s in modules],
+               os.path.join(FORTRAN, driver + ".f90"),
+               "-L" + HERE, "-lsgw_hip", "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        exes.append(exe)
+    return exes
 
 
 def build_oracle(verbose: bool = False) -> str:
