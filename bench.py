@@ -752,7 +752,9 @@ def main():
                      for k, v in per.items()}
 
     # true residuals of every (system, shift) in FP64: the last step's batch
-    # (lock-step), or every streamed batch of the timed run
+    # (lock-step), or every streamed batch of the timed run; a solve counts
+    # when it meets the requested threshold (1e-8 by default)
+    target = min(args.threshold, RES_TARGET)
     d_res = torch.zeros((nsys, nshift), dtype=torch.float64, device=dev)
     batches = ([d_x_q[b * nsys:(b + 1) * nsys] for b in range(args.steps)] if args.stream
                else [d_x])
@@ -766,7 +768,7 @@ def main():
                                                ctypes.c_void_p(d_res.data_ptr()),
                                                ctypes.c_void_p(stream)))
         res = d_res.cpu().numpy()
-        good += int((res <= RES_TARGET).sum())
+        good += int((res <= target).sum())
         res_max = max(res_max, float(res.max()))
     solves_per_s = good * (1 if args.stream else args.steps) / elapsed
 
@@ -787,10 +789,16 @@ def main():
             prof = json.load(f)
         if prof.get("workload") == key:
             traffic = prof["traffic_bytes_per_happly"]
+            kms = prof.get("happly_avg_ms")
             traffic_note = {"source": os.path.relpath(TRAFFIC_FILE, ROOT),
                             "traffic_over_algorithmic": round(traffic / b_launch, 3),
                             "measured_hbm_GBps": (round(traffic / (t_h / max(launches, 1)) / 1e9, 1)
-                                                  if t_h else None)}
+                                                  if t_h else None),
+                            # the four kernels' summed durations from the kernel
+                            # trace of the same command (no launch gaps or events)
+                            "profiled_kernel_ms": kms,
+                            "frac_over_kernel_time": (round(b_launch / (kms * 1e-3) / 1e9 / 8000.0, 4)
+                                                      if kms else None)}
     if os.path.exists(VALU_FILE):
         # FP64 VALU instructions per H-apply from rocprofv3 --pmc runs of this
         # command (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the four kernels)
@@ -804,7 +812,10 @@ def main():
                     "peak_TFLOPs": FP64_PEAK, "frac": round(rate / FP64_PEAK, 4)}
 
     out = {
-        "metric": "Sternheimer solves/s at 1e-8 residual (48^3 FFT, 32 shifts); % HBM roofline",
+        "metric": ("Sternheimer solves/s at 1e-8 residual (48^3 FFT, 32 shifts); % HBM roofline"
+                   if target == RES_TARGET else
+                   f"Sternheimer solves/s at {target:g} residual (48^3 FFT, 32 shifts); "
+                   "% HBM roofline"),
         "value": round(solves_per_s, 2),
         "unit": "solves/s",
         "n_gpus": 1,
@@ -823,7 +834,7 @@ def main():
             "npw_min_max": [int(w.npw.min()), int(w.npw.max())], "nproj": int(w.nproj),
             "threshold": args.threshold, "lmax": args.lmax, "shift_test": args.shift_test,
             "relax_threshold": relax,
-            "solves_per_step": nsys * nshift, "solves_meeting_1e-8": good,
+            "solves_per_step": nsys * nshift, f"solves_meeting_{target:g}": good,
             "solves_checked": len(batches) * nsys * nshift,
             "max_true_residual": res_max,
             "iterations_max": int(iters.max()), "iterations_median": float(np.median(iters)),

@@ -8,14 +8,14 @@
         used to scale KiB into bytes (calibration against a copy of known size)
     python -m sternheimergw_amd.profiling happly <trace.db> <fetch.db> <write.db> <copy-bytes> [skip]
         the plane-wave H-apply as one unit (all k_pw_* kernels of one
-        application): average duration per application from the kernel trace
+        application): summed duration per application from the kernel trace
         and calibrated HBM bytes read and written per application, as JSON
     python -m sternheimergw_amd.profiling sq <run_results.db> [more.db ...] [--skip N]
         per-kernel average of every counter of the databases (SQ_* and the
         like, one row per kernel and counter), then, as JSON, the FP64 VALU
         flops of one H-apply: 64 lanes x (ADD + MUL + TRANS + 2 FMA) FP64
-        instructions summed over the k_pw_* kernels (full wavefronts
-        assumed, so partial execution masks count as full)
+        instructions of all k_pw_* launches over the number of applications
+        (full wavefronts assumed, so partial execution masks count as full)
 """
 from __future__ import annotations
 
@@ -98,28 +98,40 @@ def _pmc_per_kernel(db: str, copy_bytes: int, skip_last: int = 0):
 
 def happly(trace_db: str, fetch_db: str, write_db: str, copy_bytes: int,
            skip_last: int = 0) -> dict:
-    """The H-apply's kernels launch once each per application, so the
-    application's average is the sum of the per-kernel averages.  skip_last
-    leaves out the last applications of each database (the residual checks
-    that follow the timed solve)."""
+    """The H-apply as one unit: the summed duration (and calibrated bytes) of
+    all k_pw_* kernel launches divided by the number of applications (the
+    launches of the first kernel of an application, k_pw_becp_zinv or
+    k_pw_zinv).  skip_last leaves out the last applications of each database
+    (the residual checks that follow the timed solve), i.e. the last
+    skip_last launches of every kernel that runs once per application."""
+    def per_app(rows):
+        by = collections.defaultdict(list)
+        for name, val in rows:
+            if "k_pw_" in name:
+                by[_short(name)].append(val)
+        first = [k for k in by if "k_pw_becp_zinv" in k or "k_pw_zinv" in k]
+        napp = max((len(by[k]) for k in first), default=0)
+        if skip_last:
+            by = {k: (v[:-skip_last] if len(v) >= napp and len(v) > skip_last else v)
+                  for k, v in by.items()}
+            napp -= skip_last
+        return by, max(napp, 1)
+
     con = sqlite3.connect(trace_db)
-    rows = con.execute("select name, duration from kernels order by start").fetchall()
-    dur = collections.defaultdict(list)
-    for name, d in rows:
-        if "k_pw_" in name:
-            dur[_short(name)].append(d)
-    if skip_last:
-        dur = {k: v[:-skip_last] for k, v in dur.items() if len(v) > skip_last}
-    launches = {k: len(v) for k, v in dur.items()}
-    avg_ns = {k: sum(v) / len(v) for k, v in dur.items()}
-    out = {"kernels": sorted(dur), "launches": launches,
-           "avg_us": {k: round(v / 1e3, 3) for k, v in avg_ns.items()},
-           "happly_avg_ms": round(sum(avg_ns.values()) / 1e6, 5)}
+    dur, napp = per_app(con.execute("select name, duration from kernels order by start").fetchall())
+    out = {"kernels": sorted(dur), "applications": napp,
+           "launches": {k: len(v) for k, v in dur.items()},
+           "avg_us": {k: round(sum(v) / len(v) / 1e3, 3) for k, v in dur.items()},
+           "happly_avg_ms": round(sum(sum(v) for v in dur.values()) / napp / 1e6, 5)}
     for tag, db in (("fetch", fetch_db), ("write", write_db)):
-        agg, scale = _pmc_per_kernel(db, copy_bytes, skip_last)
-        per = {_short(n): sum(v) / len(v) * 1024.0 * (scale or 1.0)
-               for n, v in agg.items() if "k_pw_" in n}
-        out[f"{tag}_factor"] = round(scale, 4) if scale else None
+        con = sqlite3.connect(db)
+        rows = con.execute("select kernel_name, value from counters_collection "
+                           "order by dispatch_id").fetchall()
+        copies = [v for n, v in rows if "copy" in n.lower() or "elementwise" in n.lower()]
+        scale = copy_bytes / (max(copies) * 1024.0) if copies and max(copies) else 1.0
+        by, n = per_app(rows)
+        per = {k: sum(v) * 1024.0 * scale / n for k, v in by.items()}
+        out[f"{tag}_factor"] = round(scale, 4)
         out[f"{tag}_bytes_per_happly"] = round(sum(per.values()))
         out[f"{tag}_bytes_per_kernel"] = {k: round(v) for k, v in per.items()}
     out["traffic_bytes_per_happly"] = out["fetch_bytes_per_happly"] + out["write_bytes_per_happly"]
@@ -144,11 +156,15 @@ def sq(dbs, skip_last: int = 0):
         lines.append(f"| `{name}` | {cname} | {len(vals)} | {sum(vals) / len(vals):.4g} |")
     weight = {"SQ_INSTS_VALU_ADD_F64": 1, "SQ_INSTS_VALU_MUL_F64": 1,
               "SQ_INSTS_VALU_TRANS_F64": 1, "SQ_INSTS_VALU_FMA_F64": 2}
+    # applications: launches of the first kernel of an H-apply
+    napp = max([len(v) for (n, c), v in agg.items()
+                if ("k_pw_becp_zinv" in n or "k_pw_zinv" in n) and c in weight] or [1])
     per_kernel = collections.defaultdict(float)
     for (name, cname), vals in agg.items():
         if "k_pw_" in name and cname in weight:
-            per_kernel[name] += 64.0 * weight[cname] * sum(vals) / len(vals)
-    out = {"fp64_flops_per_kernel": {k: round(v) for k, v in per_kernel.items()},
+            per_kernel[name] += 64.0 * weight[cname] * sum(vals) / napp
+    out = {"applications": napp,
+           "fp64_flops_per_kernel": {k: round(v) for k, v in per_kernel.items()},
            "fp64_flops_per_happly": round(sum(per_kernel.values()))}
     return "\n".join(lines), out
 
