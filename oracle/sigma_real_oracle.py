@@ -7,7 +7,7 @@ infrastructure only):
 * pade_coeff / pade_eval (algo/analytic/src/pade.f90:41-147, Vidberg and
   Serene, J. Low Temp. Phys. 29, 179 (1977)), with `protect`;
 * analytic_coeff / analytic_eval (algo/analytic/src/analytic.f90:50-348) for
-  those two models;
+  those two models and the robust Pade approximant (pade_robust_oracle.py);
 * sigma_prod (phys/corr/src/sigma.f90:417-523) on the oracle's fft6;
 * the frequency loops of sigma_correlation (sigma.f90:653-745).
 
@@ -22,8 +22,9 @@ from __future__ import annotations
 import numpy as np
 
 from . import green_oracle as go
+from . import pade_robust_oracle as pr
 
-GODBY_NEEDS, PADE_APPROX = 1, 2
+GODBY_NEEDS, PADE_APPROX, PADE_ROBUST = 1, 2, 3
 NO_SYMMETRY, EVEN_SYMMETRY, SQUARE_SYMMETRY = 0, 1, 2
 EPS8, EPS14, EPS24 = 1e-8, 1e-14, 1e-24
 
@@ -116,6 +117,11 @@ def analytic_coeff(model, w, solver, symm=NO_SYMMETRY):
     (device layout [f][igp][ig]); returns (coeff, z)."""
     if model == GODBY_NEEDS:
         return godby_needs_coeffs(np.asarray(solver)[1].imag, w[:2]), np.asarray(solver)[:2]
+    if model == PADE_ROBUST:                      # analytic.f90:158-160, no symmetrised mesh
+        solver = np.asarray(solver, dtype=np.complex128)
+        n = solver.size
+        coeff = pr.pade_coeff_robust(solver, np.moveaxis(w[:n], 0, -1))
+        return np.ascontiguousarray(np.moveaxis(coeff, -1, 0)), solver
     z, src = freqbins_symm(solver, symm)
     return pade_coeff(z, w[src]), z
 
@@ -126,6 +132,10 @@ def analytic_eval(model, coeff, z, freq, gmap, gmap_p, n, n_p):
     sub = coeff[:, np.asarray(gmap_p)[:, None], np.asarray(gmap)[None, :]]
     if model == GODBY_NEEDS:
         val = godby_needs_model(freq, sub)
+    elif model == PADE_ROBUST:
+        flat = np.moveaxis(sub, 0, -1)
+        val = np.array([[pr.pade_eval_robust(flat[i, j], freq) for j in range(flat.shape[1])]
+                        for i in range(flat.shape[0])])
     else:
         val = pade_eval(z, sub, freq)
     out[: len(gmap_p), : len(gmap)] = val

@@ -73,6 +73,7 @@ SIGMA_FP64 = 0
 SIGMA_FP32_SPLIT = 1
 GODBY_NEEDS = 1
 PADE_APPROX = 2
+PADE_ROBUST = 3
 
 # (name, restype, argtypes) of every exported entry point of include/sgw.h
 SIGNATURES = [

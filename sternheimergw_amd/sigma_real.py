@@ -1,6 +1,6 @@
 """The reference form of the correlation self-energy on the device
 (phys/corr/src/sigma.f90:528-745): W continued analytically from its solver
-frequencies (analytic.f90, Godby-Needs or Pade), then for every Sigma
+frequencies (analytic.f90: Godby-Needs, Pade or robust Pade), then for every Sigma
 frequency and every Green's-function frequency the real-space product of
 sigma_prod, accumulated into Sigma(G, G').  The GEMM restatement of the
 same sum over frequencies is sigma.py.
@@ -63,6 +63,9 @@ class ScreenedCoulomb:
         self.num_g = d_w.shape[-1]
         if model == _lib.GODBY_NEEDS:
             self.z = solver[:2].copy()
+            src = None
+        elif model == _lib.PADE_ROBUST:           # the circle itself (analytic.f90:158-160)
+            self.z = solver.copy()
             src = None
         else:
             self.z, src = freqbins_symm(solver, symm)
