@@ -426,9 +426,12 @@ int sgw_green_assemble(int32_t nfreq, int32_t n, int32_t n_p, int32_t nrow,
 #define SGW_GODBY_NEEDS 1   /* analytic.f90:33, godby_needs.f90 */
 #define SGW_PADE_APPROX 2   /* analytic.f90:36, pade.f90 (Vidberg-Serene) */
 #define SGW_PADE_ROBUST 3   /* analytic.f90:39, pade_robust.f90 (SVD-based) */
+#define SGW_AAA_APPROX 4    /* analytic.f90:42, vendor/analytic/src/aaa.f90 */
+#define SGW_AAA_POLE 5      /* analytic.f90:45, AAA with pole_correction */
 
 /* analytic_coeff (analytic.f90:50-208) in place: W on the frequency mesh
- * becomes the model's coefficients.  Godby-Needs: nfreq = 2, h_z[1] = i
+ * becomes the model's coefficients; thres is its threshold argument (tr2_gw,
+ * sigma.f90:333), used by the AAA models only.  Godby-Needs: nfreq = 2, h_z[1] = i
  * omega_p (freq%solver(2)).  Pade: the mesh of freqbins_symm, h_z[nfreq]
  * (nfreq <= 64), and h_src[i] = the slice of d_w that holds W(h_z[i]) (the
  * even-symmetry copy of freqbins_symm; NULL = i).  Robust Pade
@@ -436,9 +439,19 @@ int sgw_green_assemble(int32_t nfreq, int32_t n, int32_t n_p, int32_t nrow,
  * 10 <= nfreq <= 64 equally spaced points R e^{2 pi i f / nfreq} on a circle
  * (checked: |z_f| = R = Re z_0 within 1e-12), h_src = NULL; each element's
  * slices become [m, n, a_0..a_m, b_0..b_n, 0...] of the approximant
- * (sum a_i z^i) / (sum b_i z^i). */
-int sgw_analytic_coeff(int32_t model, int32_t num_g, int32_t nfreq, const sgw_complex *h_z,
-                       const int32_t *h_src, sgw_complex *d_w, void *stream);
+ * (sum a_i z^i) / (sum b_i z^i).  AAA (3 <= nfreq <= 64, the mesh and
+ * h_src of freqbins_symm as for Pade): SGW_AAA_APPROX keeps at most
+ * nfreq / 3 support points and stores [positions, values, weights] in
+ * thirds of nfreq / 3 slices (analytic.f90:174-193); SGW_AAA_POLE keeps the
+ * poles whose residue exceeds thres as [poles, residues] in halves of
+ * nfreq / 2 (pole_correction, analytic.f90:389-427).  Synchronises.  Errors
+ * (SGW_E_STATE, the element's slices NaN): more relevant poles than
+ * nfreq / 2 (the reference's errore), or an SGW_AAA_POLE element that is
+ * not converged at nfreq / 2 support points (the Loewner submatrix would
+ * have more columns than rows and no unique null vector). */
+int sgw_analytic_coeff(int32_t model, double thres, int32_t num_g, int32_t nfreq,
+                       const sgw_complex *h_z, const int32_t *h_src, sgw_complex *d_w,
+                       void *stream);
 
 /* analytic_eval (analytic.f90:211-348): out(nr, nr_p) = 0, then
  * out(r, c) = W_model(freq; coeff(gmap[r], gmap_p[c], :)) for r < num_g,
@@ -446,7 +459,10 @@ int sgw_analytic_coeff(int32_t model, int32_t num_g, int32_t nfreq, const sgw_co
  * the symmetrised output frequency (freq_in%symmetrize).  Robust Pade
  * (pade_eval_robust, pade_robust.f90:37-86) reads the degrees from the first
  * two slices (h_z may be NULL); an element whose degrees do not fit the
- * nfreq slices evaluates to NaN. */
+ * nfreq slices evaluates to NaN.  AAA (aaa_approx_eval, aaa_pole_eval,
+ * analytic.f90:351-459; h_z may be NULL): the barycentric form of the
+ * support points whose weight exceeds 1e-12 (a support point's value within
+ * 1e-14 of it), or the sum of residue / (freq - pole). */
 int sgw_analytic_eval(int32_t model, int32_t num_gc, int32_t nfreq, const sgw_complex *h_z,
                       const sgw_complex *d_coeff, sgw_complex freq, int32_t n, int32_t n_p,
                       int32_t num_g, int32_t num_g_p, const int32_t *d_gmap,

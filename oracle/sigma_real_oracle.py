@@ -7,7 +7,8 @@ infrastructure only):
 * pade_coeff / pade_eval (algo/analytic/src/pade.f90:41-147, Vidberg and
   Serene, J. Low Temp. Phys. 29, 179 (1977)), with `protect`;
 * analytic_coeff / analytic_eval (algo/analytic/src/analytic.f90:50-348) for
-  those two models and the robust Pade approximant (pade_robust_oracle.py);
+  those two models, the robust Pade approximant (pade_robust_oracle.py) and
+  AAA (aaa_oracle.py);
 * sigma_prod (phys/corr/src/sigma.f90:417-523) on the oracle's fft6;
 * the frequency loops of sigma_correlation (sigma.f90:653-745).
 
@@ -21,10 +22,11 @@ from __future__ import annotations
 
 import numpy as np
 
+from . import aaa_oracle as aa
 from . import green_oracle as go
 from . import pade_robust_oracle as pr
 
-GODBY_NEEDS, PADE_APPROX, PADE_ROBUST = 1, 2, 3
+GODBY_NEEDS, PADE_APPROX, PADE_ROBUST, AAA_APPROX, AAA_POLE = 1, 2, 3, 4, 5
 NO_SYMMETRY, EVEN_SYMMETRY, SQUARE_SYMMETRY = 0, 1, 2
 EPS8, EPS14, EPS24 = 1e-8, 1e-14, 1e-24
 
@@ -112,7 +114,7 @@ def godby_needs_model(freq, coeff):
     return np.where(np.abs(c1) > EPS8, res, 0.0)
 
 
-def analytic_coeff(model, w, solver, symm=NO_SYMMETRY):
+def analytic_coeff(model, w, solver, symm=NO_SYMMETRY, thres=0.0):
     """w (nfreq_sym, num_g, num_g) with the first len(solver) slices filled
     (device layout [f][igp][ig]); returns (coeff, z)."""
     if model == GODBY_NEEDS:
@@ -123,6 +125,13 @@ def analytic_coeff(model, w, solver, symm=NO_SYMMETRY):
         coeff = pr.pade_coeff_robust(solver, np.moveaxis(w[:n], 0, -1))
         return np.ascontiguousarray(np.moveaxis(coeff, -1, 0)), solver
     z, src = freqbins_symm(solver, symm)
+    if model in (AAA_APPROX, AAA_POLE):            # analytic.f90:162-199
+        vals = w[src]
+        out = np.zeros_like(vals)
+        for c in range(vals.shape[1]):
+            for r in range(vals.shape[2]):
+                out[:, c, r] = aa.aaa_coeff(model, thres, z, vals[:, c, r])
+        return out, z
     return pade_coeff(z, w[src]), z
 
 
@@ -132,6 +141,10 @@ def analytic_eval(model, coeff, z, freq, gmap, gmap_p, n, n_p):
     sub = coeff[:, np.asarray(gmap_p)[:, None], np.asarray(gmap)[None, :]]
     if model == GODBY_NEEDS:
         val = godby_needs_model(freq, sub)
+    elif model in (AAA_APPROX, AAA_POLE):
+        flat = np.moveaxis(sub, 0, -1)
+        val = np.array([[aa.aaa_eval(model, flat[i, j], freq) for j in range(flat.shape[1])]
+                        for i in range(flat.shape[0])])
     elif model == PADE_ROBUST:
         flat = np.moveaxis(sub, 0, -1)
         val = np.array([[pr.pade_eval_robust(flat[i, j], freq) for j in range(flat.shape[1])]

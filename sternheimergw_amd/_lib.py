@@ -74,6 +74,8 @@ SIGMA_FP32_SPLIT = 1
 GODBY_NEEDS = 1
 PADE_APPROX = 2
 PADE_ROBUST = 3
+AAA_APPROX = 4
+AAA_POLE = 5
 
 # (name, restype, argtypes) of every exported entry point of include/sgw.h
 SIGNATURES = [
@@ -131,8 +133,8 @@ SIGNATURES = [
                              c_int32, c_void_p, c_void_p]),
     ("sgw_green_assemble", c_int32, [c_int32, c_int32, c_int32, c_int32, c_void_p, c_int32,
                                      c_void_p, c_void_p, c_int32, c_int64, c_void_p, c_void_p]),
-    ("sgw_analytic_coeff", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, c_void_p,
-                                     c_void_p]),
+    ("sgw_analytic_coeff", c_int32, [c_int32, c_double, c_int32, c_int32, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
     ("sgw_analytic_eval", c_int32, [c_int32, c_int32, c_int32, c_void_p, c_void_p, sgw_complex,
                                     c_int32, c_int32, c_int32, c_int32, c_void_p, c_void_p,
                                     c_void_p, c_void_p]),

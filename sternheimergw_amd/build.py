@@ -23,7 +23,7 @@ FORTRAN = os.path.join(HERE, "fortran")
 OBJDIR = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["operators.hip", "bicgstab.hip", "subspace.hip", "sigma.hip", "adjoint.hip",
-               "density.hip", "boxfft.hip", "scf.hip", "fft6.hip", "analytic.hip", "ktimer.hip", "tune.hip", "mixed.hip", "capi.hip"]
+               "density.hip", "boxfft.hip", "scf.hip", "fft6.hip", "analytic.hip", "aaa.hip", "ktimer.hip", "tune.hip", "mixed.hip", "capi.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <vector>
 
+#include "aaa.h"
 #include "boxfft.h"
 #include "fft6.h"
 #include "ktimer.h"
@@ -413,6 +414,40 @@ __global__ void k_analytic_eval(int model, int N, Mesh m, double2 freq, int num_
       const double2 one = mk(1.0, 0.0);
       res = cmul(c1, cadd(cdiv(one, cadd(c2, freq)), cdiv(one, csub(c2, freq))));
     }
+  } else if (model == SGW_AAA_APPROX) {  // aaa_approx_eval (analytic.f90:351-386)
+    const int mmax = N / 3;
+    int mm = 0;
+    for (int c = 0; c < mmax; ++c)
+      if (sqrt(cabs2(coeff[(int64_t)(2 * mmax + c) * ne + el])) > 1e-12) ++mm;
+    // aaa_evaluate (aaa.f90:71-90): the barycentric form, or a support
+    // point's value within 1e-14 of it
+    double2 num = mk(0.0, 0.0), den = mk(0.0, 0.0);
+    double dmin = INFINITY;
+    int imin = 0;
+    for (int c = 0; c < mm; ++c) {
+      const double2 pc = coeff[(int64_t)c * ne + el];
+      const double2 vc = coeff[(int64_t)(mmax + c) * ne + el];
+      const double2 wc = coeff[(int64_t)(2 * mmax + c) * ne + el];
+      double2 dz = csub(freq, pc);
+      const double dist = sqrt(cabs2(dz));
+      if (dist <= 1e-14) dz = mk(1e-14, 0.0);
+      const double2 cc = cdiv(mk(1.0, 0.0), dz);
+      num = cadd(num, cmul(cc, cmul(wc, vc)));
+      den = cadd(den, cmul(cc, wc));
+      if (dist < dmin) {
+        dmin = dist;
+        imin = c;
+      }
+    }
+    res = (mm > 0 && dmin < 1e-14) ? coeff[(int64_t)(mmax + imin) * ne + el] : cdiv(num, den);
+  } else if (model == SGW_AAA_POLE) {  // aaa_pole_eval (analytic.f90:430-459)
+    const int half = N / 2;
+    int npole = 0;
+    for (int i = half; i < N; ++i)
+      if (sqrt(cabs2(coeff[(int64_t)i * ne + el])) > 0.0) ++npole;
+    for (int i = 0; i < npole && half + i < N; ++i)
+      res = cadd(res, cdiv(coeff[(int64_t)(half + i) * ne + el],
+                           csub(freq, coeff[(int64_t)i * ne + el])));
   } else if (model == SGW_PADE_ROBUST) {  // pade_eval_robust (pade_robust.f90:37-86)
     const int m = (int)lround(sqrt(cabs2(coeff[el])));
     const int n = (int)lround(sqrt(cabs2(coeff[ne + el])));
@@ -481,7 +516,7 @@ Mesh make_mesh(int N, const sgw_complex *h_z, const int32_t *h_src) {
 
 using namespace sgw;
 
-extern "C" int sgw_analytic_coeff(int32_t model, int32_t num_g, int32_t nfreq,
+extern "C" int sgw_analytic_coeff(int32_t model, double thres, int32_t num_g, int32_t nfreq,
                                   const sgw_complex *h_z, const int32_t *h_src,
                                   sgw_complex *d_w, void *stream) {
   return guarded([&] {
@@ -517,9 +552,16 @@ extern "C" int sgw_analytic_coeff(int32_t model, int32_t num_g, int32_t nfreq,
       const int gpb = std::min(kPrGroups, kPrLdsMax / per);
       k_pade_robust_coeffs<<<(unsigned)((ne + gpb - 1) / gpb), kPrLanes * gpb, (size_t)gpb * per,
                              st>>>(ne, nfreq, radius, nmax, w);
+    } else if (model == SGW_AAA_APPROX || model == SGW_AAA_POLE) {
+      // aaa_generate per element on the symmetrised mesh (analytic.f90:162-199)
+      const Mesh m = make_mesh(nfreq, h_z, h_src);
+      for (int i = 0; i < nfreq; ++i)
+        SGW_REQUIRE(m.src[i] >= 0 && m.src[i] < nfreq, SGW_E_ARG, "aaa: source slice out of range");
+      SGW_REQUIRE(thres >= 0.0, SGW_E_ARG, "aaa: threshold must be >= 0");
+      aaa_coeffs(model, thres, ne, nfreq, m.z, m.src, w, st);
     } else {
-      throw Error{SGW_E_ARG,
-                  "analytic_coeff: model must be SGW_GODBY_NEEDS, SGW_PADE_APPROX or SGW_PADE_ROBUST"};
+      throw Error{SGW_E_ARG, "analytic_coeff: model must be SGW_GODBY_NEEDS, SGW_PADE_APPROX, "
+                             "SGW_PADE_ROBUST, SGW_AAA_APPROX or SGW_AAA_POLE"};
     }
     SGW_HIP(hipGetLastError());
     return 0;
@@ -532,11 +574,17 @@ extern "C" int sgw_analytic_eval(int32_t model, int32_t num_gc, int32_t nfreq,
                                  int32_t num_g_p, const int32_t *d_gmap,
                                  const int32_t *d_gmap_p, sgw_complex *d_out, void *stream) {
   return guarded([&] {
-    SGW_REQUIRE(d_coeff && d_gmap && d_gmap_p && d_out && (h_z || model == SGW_PADE_ROBUST),
-                SGW_E_ARG, "analytic_eval: null argument");
-    SGW_REQUIRE(model == SGW_GODBY_NEEDS || model == SGW_PADE_APPROX || model == SGW_PADE_ROBUST,
-                SGW_E_ARG,
-                "analytic_eval: model must be SGW_GODBY_NEEDS, SGW_PADE_APPROX or SGW_PADE_ROBUST");
+    const bool self_contained =
+        model == SGW_PADE_ROBUST || model == SGW_AAA_APPROX || model == SGW_AAA_POLE;
+    SGW_REQUIRE(d_coeff && d_gmap && d_gmap_p && d_out && (h_z || self_contained), SGW_E_ARG,
+                "analytic_eval: null argument");
+    SGW_REQUIRE(model >= SGW_GODBY_NEEDS && model <= SGW_AAA_POLE, SGW_E_ARG,
+                "analytic_eval: model must be SGW_GODBY_NEEDS, SGW_PADE_APPROX, SGW_PADE_ROBUST, "
+                "SGW_AAA_APPROX or SGW_AAA_POLE");
+    SGW_REQUIRE(model != SGW_AAA_APPROX || (nfreq >= 3 && nfreq <= kMaxPade), SGW_E_SHAPE,
+                "analytic_eval: AAA coefficients need 3..64 slices");
+    SGW_REQUIRE(model != SGW_AAA_POLE || (nfreq >= 2 && nfreq <= kMaxPade), SGW_E_SHAPE,
+                "analytic_eval: AAA pole coefficients need 2..64 slices");
     SGW_REQUIRE(model != SGW_PADE_ROBUST || (nfreq >= 4 && nfreq <= kMaxPade), SGW_E_SHAPE,
                 "analytic_eval: robust Pade coefficients need 4..64 slices");
     SGW_REQUIRE(model != SGW_GODBY_NEEDS || nfreq == 2, SGW_E_SHAPE,
@@ -544,7 +592,7 @@ extern "C" int sgw_analytic_eval(int32_t model, int32_t num_gc, int32_t nfreq,
     const int64_t nr = (int64_t)n * n * n, nr_p = (int64_t)n_p * n_p * n_p;
     SGW_REQUIRE(num_g >= 1 && num_g <= nr && num_g_p >= 1 && num_g_p <= nr_p, SGW_E_SHAPE,
                 "analytic_eval: need num_g <= n^3 and num_g_p <= n_p^3");
-    const Mesh m = model == SGW_PADE_ROBUST ? Mesh{} : make_mesh(nfreq, h_z, nullptr);
+    const Mesh m = self_contained ? Mesh{} : make_mesh(nfreq, h_z, nullptr);
     hipStream_t st = (hipStream_t)stream;
     double2 *out = reinterpret_cast<double2 *>(d_out);
     KScope ks("analytic_eval", st);

@@ -1,6 +1,6 @@
 """The reference form of the correlation self-energy on the device
 (phys/corr/src/sigma.f90:528-745): W continued analytically from its solver
-frequencies (analytic.f90: Godby-Needs, Pade or robust Pade), then for every Sigma
+frequencies (analytic.f90: Godby-Needs, Pade, robust Pade or AAA), then for every Sigma
 frequency and every Green's-function frequency the real-space product of
 sigma_prod, accumulated into Sigma(G, G').  The GEMM restatement of the
 same sum over frequencies is sigma.py.
@@ -54,10 +54,12 @@ class ScreenedCoulomb:
     """Coefficients of W's analytic continuation on the device
     (analytic_coeff, analytic.f90:50-208)."""
 
-    def __init__(self, d_w, solver, model: int = _lib.PADE_APPROX, symm: int = NO_SYMMETRY):
+    def __init__(self, d_w, solver, model: int = _lib.PADE_APPROX, symm: int = NO_SYMMETRY,
+                 thres: float = 0.0):
         """d_w: (nfreq_sym, num_g, num_g) complex128 device tensor with W at
         the solver frequencies in its first len(solver) slices; overwritten
-        by the coefficients."""
+        by the coefficients.  thres: analytic_coeff's threshold (tr2_gw),
+        used by the AAA models."""
         solver = np.asarray(solver, dtype=np.complex128)
         self.model, self.symm = model, symm
         self.num_g = d_w.shape[-1]
@@ -76,8 +78,8 @@ class ScreenedCoulomb:
         hz = ctypes.c_void_p(self._hz_buf.ctypes.data)
         self._src_buf = None if src is None else np.ascontiguousarray(src, dtype=np.int32)
         hs = None if src is None else ctypes.c_void_p(self._src_buf.ctypes.data)
-        _lib.check(_lib.lib().sgw_analytic_coeff(model, self.num_g, nz, hz, hs, _p(d_w),
-                                                 _stream()))
+        _lib.check(_lib.lib().sgw_analytic_coeff(model, float(thres), self.num_g, nz, hz, hs,
+                                                 _p(d_w), _stream()))
         self._hz = hz
         self.coeff = d_w
 
