@@ -80,14 +80,22 @@ def parse():
                     help="after the timed steps, one untimed step with per-kernel events")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device copy of this size (MiB) for counter calibration")
-    ap.add_argument("--workload", choices=("solve", "sigma", "response", "sigma_real", "mixed"),
+    ap.add_argument("--workload", choices=("solve", "sigma", "response", "sigma_real", "mixed",
+                                           "continuation"),
                     default="solve",
                     help="solve: the batched Sternheimer solve (headline); sigma: the Sigma_c "
                          "frequency convolution GEMM (N_G = 1500, 64 x 64 frequencies); "
                          "response: one dielectric column (direct solve_linter, 48^3); "
                          "sigma_real: the reference-form Sigma_c products (24^3, N_G 1500); "
                          "mixed: FP32 FFTs with FP64 refinement against FP64 (24^3, chi0 "
-                         "columns at N_G 500)")
+                         "columns at N_G 500); continuation: the analytic continuation "
+                         "coefficients of W (robust Pade, AAA, AAA poles; N_G 500)")
+    ap.add_argument("--cont-model", choices=("pade_robust", "aaa", "aaa_pole"),
+                    default="pade_robust",
+                    help="the model the continuation workload's value is quoted on (all three "
+                         "are timed)")
+    ap.add_argument("--cont-ng", type=int, default=500,
+                    help="N_G of the continuation workload (N_G^2 elements)")
     ap.add_argument("--sigma-precision", choices=("fp64", "fp32s"), default="fp64",
                     help="fp32s: FP32 MFMA products, per-frequency FP32 sums, FP64 frequency sum")
     ap.add_argument("--sigma-batch", type=int, default=8,
@@ -597,6 +605,108 @@ def cpu_baseline(w, threshold, lmax, gpu_iters, shift_test, per_core=2):
     }
 
 
+def run_continuation(args):
+    """analytic_coeff (analytic.f90:50-208) on W(G, G') of N_G^2 elements:
+    robust Pade from a 32-point circle of radius 1 (W a sum of 2-4 poles
+    outside radius 2), AAA and AAA with poles from the 35 imaginary
+    frequencies of the reference's GW example (W a sum of 1-4 poles off the
+    axis plus noise at 1e-12 of the element's largest value, threshold 1e-10).  A step is one call per model;
+    the value is the chosen model's elements/s."""
+    import torch
+
+    from oracle import aaa_oracle as aa
+    from oracle import pade_robust_oracle as pro
+    from sternheimergw_amd import _lib, sigma_real
+
+    L = _lib.lib()
+    dev = torch.device("cuda:0")
+    ng = args.cont_ng
+    ne = ng * ng
+    rng = np.random.default_rng(11)
+    gold = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests",
+                                       "golden", "aaa_reference.json")))
+    circle = np.exp(2j * np.pi * np.arange(32) / 32)
+    axis = 1j * np.array(gold["realistic"]["z_imag"])
+
+    def poles_on(z, outside):
+        npole = 4
+        p = (rng.uniform(2.0, 4.0, (npole, ne)) * np.exp(2j * np.pi * rng.random((npole, ne)))
+             if outside else rng.standard_normal((npole, ne)) * 2.0
+             + 1j * rng.standard_normal((npole, ne)) * 0.5
+             + np.sign(rng.standard_normal((npole, ne))) * 0.5)
+        keep = rng.random((npole, ne)) < 0.75
+        keep[0] = True
+        c = (rng.standard_normal((npole, ne)) + 1j * rng.standard_normal((npole, ne))) * keep
+        out = np.zeros((z.size, ne), np.complex128)
+        for i in range(npole):
+            out += c[i][None, :] / (z[:, None] - p[i][None, :])
+        return out.reshape(z.size, ng, ng)
+
+    data = {"pade_robust": (circle, poles_on(circle, True), _lib.PADE_ROBUST, 0.0)}
+    w_axis = poles_on(axis, False)
+    scale = np.abs(w_axis).max(axis=0, keepdims=True)
+    w_axis += 1e-12 * scale * (rng.standard_normal(w_axis.shape)
+                               + 1j * rng.standard_normal(w_axis.shape))
+    data["aaa"] = (axis, w_axis, _lib.AAA_APPROX, 1e-10)
+    data["aaa_pole"] = (axis, w_axis, _lib.AAA_POLE, 1e-10)
+    times = {}
+    for name, (z, W, model, thres) in data.items():
+        host = torch.from_numpy(np.ascontiguousarray(W)).to(dev)
+        d_w = torch.empty_like(host)
+        for _ in range(args.warmup):
+            d_w.copy_(host)
+            sigma_real.ScreenedCoulomb(d_w, z, model, 0, thres=thres)
+        torch.cuda.synchronize()
+        el = 0.0
+        for s in range(args.steps):
+            d_w.copy_(host)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            sigma_real.ScreenedCoulomb(d_w, z, model, 0, thres=thres)
+            torch.cuda.synchronize()
+            el += time.perf_counter() - t0
+        times[name] = el / args.steps
+        print(f"{name}: {ne / times[name]:.0f} elements/s ({times[name] * 1e3:.1f} ms per call)",
+              flush=True)
+    name = args.cont_model
+    z, W, model, thres = data[name]
+    t = times[name]
+    nbytes = 2.0 * 16 * z.size * ne            # read and write every element's slices
+    out = {
+        "metric": f"W elements continued/s ({name}, analytic_coeff, N_G {ng})",
+        "value": round(ne / t, 1), "unit": "elements/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sum(times.values()) * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "data": "synthetic (W as sums of poles on the model's mesh)",
+        "config": {"workload": f"continuation-{name}: N_G {ng} ({ne} elements), "
+                               f"{z.size} frequencies", "num_g": ng, "nfreq": int(z.size)},
+        "per_model_elements_per_s": {k: round(ne / v, 1) for k, v in times.items()},
+        "roofline": {"kernel": {"pade_robust": "k_pade_robust_coeffs", "aaa": "k_aaa_coeffs",
+                                "aaa_pole": "k_aaa_coeffs"}[name],
+                     "bound": "hbm", "achieved": round(nbytes / t / 1e9, 3), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(nbytes / t / 1e9 / 8000.0, 6),
+                     "traffic": None, "algorithmic_bytes_per_launch": nbytes,
+                     "note": "per-element SVD/QR and Aberth work in one wavefront (half a "
+                             "wavefront for robust Pade) bounded by the group reductions' "
+                             "latency, not by HBM; the fraction says so"},
+    }
+    if args.cpu_baseline:
+        ns = {"pade_robust": 10000, "aaa": 5000, "aaa_pole": 300}[name]   # ~1.5-2 s each
+        flat = W.reshape(z.size, ne)
+        tc = time.perf_counter()
+        for e in range(ns):
+            if name == "pade_robust":
+                pro.pade_coeff_robust(z, flat[:, e][None, :])
+            else:
+                aa.aaa_coeff(4 if name == "aaa" else 5, thres, z, flat[:, e])
+        tc = time.perf_counter() - tc
+        out["cpu_baseline"] = {"value": round(ns / tc, 2), "unit": "elements/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{ns} elements through the oracle (numpy LAPACK SVD/QR/"
+                                         f"eig per element) on one core, {tc:.2f} s"}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -614,6 +724,9 @@ def main():
         return
     if args.workload == "mixed":
         run_mixed(args)
+        return
+    if args.workload == "continuation":
+        run_continuation(args)
         return
     import torch
 
