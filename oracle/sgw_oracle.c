@@ -147,6 +147,7 @@ typedef struct {
   ozc sigma, inv_phi_old, inv_phi, inv_phi_new, inv_theta, alpha, beta;
   ozc *mu;        /* l(l+1)/2, 1-based index ij -> [ij-1] */
   ozc *gamma, *gamma_p, *gamma_pp;
+  int frozen;     /* shift test: done at the end of an earlier iteration, vectors kept */
 } shift_t;
 
 #define U(s, i) ((s)->uu + (int64_t)(i) * n)
@@ -235,15 +236,19 @@ static void bicg_part(int l, oracle_op_fn op, void *ctx, int64_t n, seed_t *s,
       a->beta = zmul(zmul(q, q), s->beta);
       a->alpha = zmul(zdiv(a->inv_phi_new, a->inv_phi), s->alpha);
       ozc factor = zmul(a->inv_theta, a->inv_phi);
-      for (int ii = 0; ii <= jj; ++ii) {                               /* L14-15 */
-        zscal(n, zneg(a->beta), U(a, ii));
-        zaxpy(n, factor, R(s, ii), U(a, ii));
+      if (!a->frozen) {
+        for (int ii = 0; ii <= jj; ++ii) {                             /* L14-15 */
+          zscal(n, zneg(a->beta), U(a, ii));
+          zaxpy(n, factor, R(s, ii), U(a, ii));
+        }
+        zaxpy(n, a->alpha, U(a, 0), a->xx);                            /* L17 */
       }
-      zaxpy(n, a->alpha, U(a, 0), a->xx);                              /* L17 */
       a->inv_phi_old = a->inv_phi;                                     /* L18 */
       a->inv_phi = a->inv_phi_new;
-      memcpy(U(a, jj + 1), R(s, jj), sizeof(ozc) * n);                 /* L19 */
-      zscal(n, factor, U(a, jj + 1));
+      if (!a->frozen) {
+        memcpy(U(a, jj + 1), R(s, jj), sizeof(ozc) * n);               /* L19 */
+        zscal(n, factor, U(a, jj + 1));
+      }
     }
     s->alpha_old = s->alpha;                                           /* L18 */
     for (int ii = 0; ii <= jj; ++ii)                                   /* L21-22 */
@@ -252,6 +257,7 @@ static void bicg_part(int l, oracle_op_fn op, void *ctx, int64_t n, seed_t *s,
     zaxpy(n, s->alpha, U(s, 0), s->xx);                                /* L25 */
     for (int k = 0; k < nsh; ++k) {                                    /* L26 */
       shift_t *a = &sh[k];
+      if (a->frozen) continue;
       ozc factor = zneg(zmul(a->inv_theta, a->inv_phi));
       zaxpy(n, factor, R(s, jj), U(a, jj + 1));
       zscal(n, zdiv(ONE, a->alpha), U(a, jj + 1));
@@ -332,6 +338,7 @@ static void mr_part(int l, int64_t n, seed_t *s, shift_t *sh, int nsh) {
         G(a->gamma_pp, jj) = zadd(G(a->gamma_pp, jj), zmul(TAU(ij), G(a->gamma_p, ii + 1)));
       }
     }
+    if (a->frozen) continue;
     ozc factor = zmul(G(a->gamma_p, 1), inv_xi);                       /* L28 */
     zaxpy(n, factor, R(s, 0), a->xx);
     zaxpy(n, zneg(G(s->gamma, l)), U(a, l), U(a, 0));
@@ -353,15 +360,19 @@ static void mr_part(int l, int64_t n, seed_t *s, shift_t *sh, int nsh) {
 
 /* Convergence test of bicgstab.f90:296-299 on the seed residual; with
  * shift_test (an extension, no counterpart in the reference) the largest
- * shifted residual estimate |1/(theta phi)| ||r_seed|| must pass as well. */
-static int converged(int64_t n, const seed_t *s, const shift_t *sh, int nsh, double threshold,
-                     int shift_test) {
+ * shifted residual estimate |1/(theta phi)| ||r_seed|| must pass as well.
+ * At the end of an iteration (end_of_iter) a shifted system whose estimate
+ * passes is done: its vectors are kept from then on and it leaves the test. */
+static int converged(int64_t n, const seed_t *s, shift_t *sh, int nsh, double threshold,
+                     int shift_test, int end_of_iter) {
   double nr = oracle_norm(s->rr, n), zmax = 1.0;
   if (shift_test)
     for (int k = 0; k < nsh; ++k) {
+      if (sh[k].frozen) continue;
       ozc z = zmul(sh[k].inv_theta, sh[k].inv_phi);
       double a = sqrt(z.re * z.re + z.im * z.im);
-      if (a > zmax) zmax = a;
+      if (end_of_iter && a * nr < threshold) sh[k].frozen = 1;
+      else if (a > zmax) zmax = a;
     }
   return nr * zmax < threshold;
 }
@@ -385,9 +396,9 @@ int oracle_bicgstab_ex(int lmax, double threshold, int max_iter, oracle_op_fn op
   init_shift(l, n, sigma, nshift, sh);
   for (iter = 1; iter <= max_iter; ++iter) {
     bicg_part(l, op, ctx, n, &s, sh, nsh);
-    if (converged(n, &s, sh, nsh, threshold, shift_test)) break;
+    if (converged(n, &s, sh, nsh, threshold, shift_test, 0)) break;
     mr_part(l, n, &s, sh, nsh);
-    if (converged(n, &s, sh, nsh, threshold, shift_test)) break;
+    if (converged(n, &s, sh, nsh, threshold, shift_test, 1)) break;
   }
   if (iter > max_iter) {
     printf("WARNING: BiCGstab algorithm did not converge in %d iterations.\n", max_iter);

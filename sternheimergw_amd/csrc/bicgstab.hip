@@ -101,6 +101,10 @@ struct Scal {
   double2 *fa, *fb, *ia;       // factors for L15/L19, L26 and 1/alpha^sigma
   double2 *mu;                 // [S*nsh][l(l+1)/2]
   double2 *coef;               // [S*nsh][l+2]: gamma'_1/xi, gamma''_j/xi (j=1..l-1), 1/psi
+  // with the per-shift test: a shifted system whose residual estimate
+  // |zeta_k| ||r_seed|| fell below the threshold at the end of an iteration
+  // keeps its x^sigma from then on (k_shift_cycle skips it)
+  int32_t *frozen;             // [S*nsh]
 };
 
 // bicgstab.f90:595-599  rho = (r_j, r0~); beta = alpha rho / rho_old; rho_old = rho.
@@ -282,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_shift_cycle(const int32_t *__restric
   const double2 *gam = sc.gamma + s * l;
   for (int k = kg; k < v.nsh; k += groups) {
     const int idx = s * v.nsh + k;
+    if (sc.frozen[idx]) continue;  // uniform over the wavefront (one group per wavefront)
     // u_1..u_l are set by L19/L26 of steps 0..l-1 before any use, so only
     // u_0 and x carry over from the previous iteration
     double2 u[L + 1];
@@ -496,8 +501,13 @@ __global__ void k_check(int nsys, const int32_t *__restrict__ count, int max_ite
   if (shift_test) {
     double zmax = 1.0;
     for (int k = 0; k < nsh; ++k) {
+      if (sc.frozen[s * nsh + k]) continue;
       const double2 z = cmul(sc.inv_theta[s * nsh + k], sc.inv_phi[s * nsh + k]);
-      zmax = fmax(zmax, sqrt(z.x * z.x + z.y * z.y));
+      const double az = sqrt(z.x * z.x + z.y * z.y);
+      // at the end of an iteration x^sigma holds all of its updates: a
+      // shifted system below the threshold is done
+      if (final && az * nr < thr) sc.frozen[s * nsh + k] = 1;
+      else zmax = fmax(zmax, az);
     }
     nr *= zmax;
   }
@@ -558,6 +568,7 @@ __device__ void init_shift_one(int s, int q, int k, int nsh, int l, const Scal &
   sc.inv_phi_old[idx] = one;
   sc.inv_phi[idx] = one;
   sc.inv_theta[idx] = one;
+  sc.frozen[idx] = 0;
   double2 sg = csub(sigma_all[(int64_t)q * nshift_all + k + 1], s1);
   sc.sigma[idx] = sg;
   double2 spow[kMaxL + 1];
@@ -660,6 +671,7 @@ __global__ void k_move_scalars(int npairs, const int32_t *__restrict__ pairs, in
     const int is = s * nsh + k, id = d * nsh + k;
     sc.sigma[id] = sc.sigma[is];
     sc.inv_phi_old[id] = sc.inv_phi_old[is];
+    sc.frozen[id] = sc.frozen[is];
     sc.inv_phi[id] = sc.inv_phi[is];
     sc.inv_phi_new[id] = sc.inv_phi_new[is];
     sc.inv_theta[id] = sc.inv_theta[is];
@@ -779,7 +791,7 @@ struct BatchBicgstab::Impl {
   DevBuf<double2> scal_seed, scal_shift, mu, coef, gam;
   DevBuf<double2> sigma_seed;  // per system seed shift sigma_1
   DevBuf<double> norms;
-  DevBuf<int32_t> active, mr_active, iters, state, count;
+  DevBuf<int32_t> active, mr_active, iters, state, count, frozen;
   Scal sc{};
   Vecs vv{};
   std::vector<double2> h_sigma_seed;
@@ -926,6 +938,9 @@ struct BatchBicgstab::Impl {
     sc.ia = qs + 4 * nstep;
     sc.mu = mu.p;
     sc.coef = coef.p;
+    frozen.alloc(nshs);
+    SGW_HIP(hipMemset(frozen.p, 0, sizeof(int32_t) * nshs));
+    sc.frozen = frozen.p;
     vv.R = R.p; vv.U = U.p; vv.X = X.p; vv.RT = RT.p; vv.SU = SU.p; vv.SX = SX.p; vv.RS = RS.p;
     vv.ld = ld; vv.l = l; vv.nsh = nsh; vv.nsnap = nsnap;
     SGW_HIP(hipHostMalloc((void **)&pinned_active, sizeof(int32_t) * 2 * S, hipHostMallocDefault));

@@ -84,6 +84,31 @@ def test_shift_test_covers_every_shift(lin_prob):
     assert res[True].max() < thr
 
 
+def test_shift_test_keeps_converged_shifts(lin_prob):
+    """Under the shift test a shifted system that passes at the end of an
+    iteration keeps its solution: each shift's x is, bit for bit, the
+    solution of some iteration t of the same solve without any stopping
+    test, several shifts stop before the last iteration, and every true
+    shifted residual is below the threshold."""
+    a, b = lin_prob["LINEAR_OPERATOR"], lin_prob["RIGHT_HAND_SIDE"]
+    sig = lin_prob["LIST_SHIFT"][::-1].copy()
+    thr = 1e-8
+    op = orc.DenseOp(a)
+    x, ierr, it = orc.bicgstab(op, b, sig, lmax=4, threshold=thr, shift_test=True)
+    assert ierr == 0
+    res = np.array([np.linalg.norm(b - (a @ x[k] + sig[k] * x[k])) for k in range(sig.size)])
+    assert res.max() < thr
+    when = np.full(sig.size, -1)
+    for t in range(1, it + 1):
+        xt, _, _ = orc.bicgstab(op, b, sig, lmax=4, threshold=1e-300, max_iter=t)
+        for k in range(sig.size):
+            if when[k] < 0 and np.array_equal(x[k], xt[k]):
+                when[k] = t
+    assert np.all(when > 0)
+    assert when[0] == it                       # the seed runs to the end
+    assert (when < it).sum() >= 10, when
+
+
 @pytest.mark.parametrize("ll", [1, 2, 4, 7])
 def test_bicgstab_collinear_residuals(ll):
     """Frommer's invariant: every shifted residual b - (A + sigma_k) x_k stays
