@@ -108,8 +108,8 @@ class PwOp:
 def bicgstab(op, b, sigma, lmax=4, threshold=1e-4, max_iter=10000, shift_test=False):
     """Returns x (nshift, n), ierr, iterations.  shift_test=True also waits
     for every shifted residual estimate (an extension of the reference's
-    test); a shifted system that passes at the end of an iteration keeps its
-    solution from then on."""
+    test); a shifted system whose estimate is below half the threshold at the
+    end of an iteration keeps its solution from then on."""
     b = np.ascontiguousarray(b, dtype=np.complex128)
     sigma = np.ascontiguousarray(np.atleast_1d(sigma), dtype=np.complex128)
     x = np.zeros((sigma.size, b.size), dtype=np.complex128)

@@ -362,7 +362,9 @@ static void mr_part(int l, int64_t n, seed_t *s, shift_t *sh, int nsh) {
  * shift_test (an extension, no counterpart in the reference) the largest
  * shifted residual estimate |1/(theta phi)| ||r_seed|| must pass as well.
  * At the end of an iteration (end_of_iter) a shifted system whose estimate
- * passes is done: its vectors are kept from then on and it leaves the test. */
+ * is below half the threshold is done: its vectors are kept from then on and
+ * it leaves the test (the half leaves room for the gap between the estimate
+ * and the true residual). */
 static int converged(int64_t n, const seed_t *s, shift_t *sh, int nsh, double threshold,
                      int shift_test, int end_of_iter) {
   double nr = oracle_norm(s->rr, n), zmax = 1.0;
@@ -371,7 +373,7 @@ static int converged(int64_t n, const seed_t *s, shift_t *sh, int nsh, double th
       if (sh[k].frozen) continue;
       ozc z = zmul(sh[k].inv_theta, sh[k].inv_phi);
       double a = sqrt(z.re * z.re + z.im * z.im);
-      if (end_of_iter && a * nr < threshold) sh[k].frozen = 1;
+      if (end_of_iter && a * nr < 0.5 * threshold) sh[k].frozen = 1;
       else if (a > zmax) zmax = a;
     }
   return nr * zmax < threshold;

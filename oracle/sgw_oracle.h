@@ -47,8 +47,8 @@ int oracle_bicgstab(int lmax, double threshold, int max_iter,
 
 /* oracle_bicgstab with shift_test = 1: a solve also waits for every shifted
  * residual estimate |1/(theta phi)| ||r_seed|| to pass (an extension; 0 is
- * the reference's test).  A shifted system whose estimate passes at the end
- * of an iteration keeps its vectors from then on. */
+ * the reference's test).  A shifted system whose estimate is below half the
+ * threshold at the end of an iteration keeps its vectors from then on. */
 int oracle_bicgstab_ex(int lmax, double threshold, int max_iter,
                        oracle_op_fn op, void *ctx, const ozc *b, int64_t n,
                        const ozc *sigma, int nshift, ozc *x, int *iters_out, int shift_test);

@@ -102,8 +102,8 @@ struct Scal {
   double2 *mu;                 // [S*nsh][l(l+1)/2]
   double2 *coef;               // [S*nsh][l+2]: gamma'_1/xi, gamma''_j/xi (j=1..l-1), 1/psi
   // with the per-shift test: a shifted system whose residual estimate
-  // |zeta_k| ||r_seed|| fell below the threshold at the end of an iteration
-  // keeps its x^sigma from then on (k_shift_cycle skips it)
+  // |zeta_k| ||r_seed|| fell below half the threshold at the end of an
+  // iteration keeps its x^sigma from then on (k_shift_cycle skips it)
   int32_t *frozen;             // [S*nsh]
 };
 
@@ -505,8 +505,9 @@ __global__ void k_check(int nsys, const int32_t *__restrict__ count, int max_ite
       const double2 z = cmul(sc.inv_theta[s * nsh + k], sc.inv_phi[s * nsh + k]);
       const double az = sqrt(z.x * z.x + z.y * z.y);
       // at the end of an iteration x^sigma holds all of its updates: a
-      // shifted system below the threshold is done
-      if (final && az * nr < thr) sc.frozen[s * nsh + k] = 1;
+      // shifted system below half the threshold is done (the half leaves
+      // room for the gap between this estimate and the true residual)
+      if (final && az * nr < 0.5 * thr) sc.frozen[s * nsh + k] = 1;
       else zmax = fmax(zmax, az);
     }
     nr *= zmax;

@@ -85,8 +85,9 @@ def test_shift_test_covers_every_shift(lin_prob):
 
 
 def test_shift_test_keeps_converged_shifts(lin_prob):
-    """Under the shift test a shifted system that passes at the end of an
-    iteration keeps its solution: each shift's x is, bit for bit, the
+    """Under the shift test a shifted system whose estimate is below half the
+    threshold at the end of an iteration keeps its solution: each shift's x
+    is, bit for bit, the
     solution of some iteration t of the same solve without any stopping
     test, several shifts stop before the last iteration, and every true
     shifted residual is below the threshold."""
