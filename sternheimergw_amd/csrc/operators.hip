@@ -186,6 +186,12 @@ template <typename C> constexpr int kSlotScale = (int)(sizeof(double2) / sizeof(
 __device__ inline bool precision_skip(const int32_t *__restrict__ relax, int want, int v) {
   return relax != nullptr && relax[v] != want;
 }
+// RL false: a kernel instance that never reads relax flags, so it compiles to
+// the same code (and rounds the same) as one without them
+template <bool RL>
+__device__ inline bool precision_skip(const int32_t *__restrict__ relax, int want, int v) {
+  return RL && precision_skip(relax, want, v);
+}
 
 // dst[c] = src[c], c < n, by one wavefront: 16 loads per lane are issued
 // before the first LDS store, so a plane of up to 1024 column values costs
@@ -301,35 +307,38 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
 // pair take the same early exits.  MB >= 0: the sphere's z half-width M3 as
 // a compile-time constant, so values outside |m3| <= MB are literal zeros
 // (inverse) or never stored (forward); -1: no such bound.
-template <int N, int MB>
+template <int N, int MB, typename C = double2, bool RL = false>
 __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict__ in,
                                        int64_t sin, const int32_t *__restrict__ group,
-                                       const int32_t *__restrict__ active,
-                                       double2 *__restrict__ T, int v, int c, bool odd) {
+                                       const int32_t *__restrict__ active, C *__restrict__ T,
+                                       int v, int c, bool odd,
+                                       const int32_t *__restrict__ relax = nullptr,
+                                       int want = 0) {
   constexpr int H = N / 2, Q = N / 4;
   if (active && !active[v]) return;
+  if (precision_skip<RL>(relax, want, v)) return;
   if (c >= d.nslot) return;
   const int g = group ? group[v] : 0;
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
-  double2 *Tv = T + (int64_t)v * N * d.nslot + c;
+  C *Tv = T + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
   const int zo = odd ? Q : 0;  // this lane's z: zo + j and H + zo + j
   if (cnt == 0) {
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
-      Tv[(int64_t)(zo + j) * d.nslot] = mk(0, 0);
-      Tv[(int64_t)(H + zo + j) * d.nslot] = mk(0, 0);
+      Tv[(int64_t)(zo + j) * d.nslot] = czero<C>();
+      Tv[(int64_t)(H + zo + j) * d.nslot] = czero<C>();
     }
     return;
   }
   const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
   const double2 *x = in + v * sin;
-  double2 b[H];
+  C b[H];
 #pragma unroll
   for (int n = 0; n < H; ++n) {
     const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k = m - lo;
     const bool band = MB < 0 || (m >= -MB && m <= MB);
-    b[n] = (band && k >= 0 && k < cnt) ? x[sidx[k]] : mk(0, 0);
+    b[n] = (band && k >= 0 && k < cnt) ? to_c<C>(x[sidx[k]]) : czero<C>();
   }
   split_dit<N, +1>(b, odd);
 #pragma unroll
@@ -339,24 +348,27 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
   }
 }
 
-template <int N, int MB>
-__global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const double2 *__restrict__ S,
+template <int N, int MB, typename C = double2, bool RL = false>
+__global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const C *__restrict__ S,
                                                       const int32_t *__restrict__ group,
                                                       const int32_t *__restrict__ active,
-                                                      double2 *__restrict__ loc) {
+                                                      double2 *__restrict__ loc,
+                                                      const int32_t *__restrict__ relax = nullptr,
+                                                      int want = 0) {
   constexpr int H = N / 2, Q = N / 4;
   const int v = blockIdx.y, c = blockIdx.x * 32 + (int)(threadIdx.x >> 1);
   const bool odd = (threadIdx.x & 1) != 0;
   if (active && !active[v]) return;
+  if (precision_skip<RL>(relax, want, v)) return;
   if (c >= d.nslot) return;
   const int g = group ? group[v] : 0;
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
   if (cnt == 0) return;
   const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
-  const double2 *Sv = S + (int64_t)v * N * d.nslot + c;
+  const C *Sv = S + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
   const int zo = odd ? Q : 0;
-  double2 b[H];
+  C b[H];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
     b[j] = Sv[(int64_t)(zo + j) * d.nslot];
@@ -369,7 +381,7 @@ __global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const double2 *__
   for (int k = 0; k < H; ++k) {
     const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N, k3 = m - lo;
     const bool band = MB < 0 || (m >= -MB && m <= MB);
-    if (band && k3 >= 0 && k3 < cnt) lv[sidx[k3]] = mk(b[k].x * sc, b[k].y * sc);
+    if (band && k3 >= 0 && k3 < cnt) lv[sidx[k3]] = mk((double)b[k].x * sc, (double)b[k].y * sc);
   }
 }
 
@@ -579,12 +591,14 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
 // that only touch those (the file is built without signed zeros and
 // non-finite values, which lets x + 0 and x * 0 fold).  MB = -1: run-time
 // half-widths.
-template <int N, int MB, int PL, int W>
-__device__ inline void plane_split_body(const PwDev &d, const double2 *__restrict__ Tin,
+template <int N, int MB, int PL, int W, typename C = double2, bool RL = false>
+__device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Tin,
                                         const int32_t *__restrict__ group,
                                         const int32_t *__restrict__ active,
-                                        double2 *__restrict__ S, int z, int v0, int nvec,
-                                        double2 *smem) {
+                                        C *__restrict__ S, int z, int v0, int nvec, C *smem,
+                                        const int32_t *__restrict__ relax = nullptr,
+                                        int want = 0) {
+  using R = typename CplxT<C>::S;
   constexpr int H = N / 2, Q = N / 4, NP = 32 * W;  // NP lane pairs
   constexpr int PN = N + 1;  // row stride of P: rows of N values would share LDS banks
   const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
@@ -594,31 +608,32 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
   bool live[PL], any = false;
 #pragma unroll
   for (int p = 0; p < PL; ++p) {
-    live[p] = v0 + p < nvec && (!active || active[v0 + p]);
+    live[p] = v0 + p < nvec && (!active || active[v0 + p]) &&
+              !precision_skip<RL>(relax, want, v0 + p);
     any |= live[p];
   }
   if (!any) return;
-  double2 *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][PN]
+  C *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][PN]
 #pragma unroll
   for (int p = 0; p < PL; ++p)  // plane p staged by wavefront p mod W
     if ((t >> 6) == p % W && live[p])
-      stage_plane(Tin + ((int64_t)(v0 + p) * N + z) * nslot, T + p * nslot, nslot);
+      stage_plane(Tin + ((int64_t)(v0 + p) * N * kSlotScale<C> + z) * nslot, T + p * nslot, nslot);
   __syncthreads();
   const int yp = r / nc1, a = r - yp * nc1;
   const bool yrow = yp < PL && live[yp];
-  double2 b[H];
+  C b[H];
   if (yrow) {
-    const double2 *Tp = T + yp * nslot + a * nc2 + M2;
+    const C *Tp = T + yp * nslot + a * nc2 + M2;
 #pragma unroll
     for (int n = 0; n < H; ++n) {
       const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      b[n] = (m >= -M2 && m <= M2) ? Tp[m] : mk(0, 0);
+      b[n] = (m >= -M2 && m <= M2) ? Tp[m] : czero<C>();
     }
   }
   __syncthreads();  // every column value is in registers before P overwrites T
   if (yrow) {  // Y inverse of row a
     split_dit_sel<N, +1>(b, odd);
-    double2 *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
+    C *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       Pr[j] = b[j];
@@ -638,19 +653,19 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
       __asm__ volatile("" : "+v"(y)::"memory");
     else
       __asm__ volatile("" : "+v"(y), "+s"(M1)::"memory");
-    double2 *Pp = P + p * nc1 * PN + y;
+    C *Pp = P + p * nc1 * PN + y;
 #pragma unroll
     for (int n = 0; n < H; ++n) {
       const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * PN] : mk(0, 0);
+      b[n] = (m >= -M1 && m <= M1) ? Pp[(m + M1) * PN] : czero<C>();
     }
     split_dit_sel<N, +1>(b, odd);
     const double *vcol = d.vloc_zxy + ((int64_t)z * N + (odd ? Q : 0)) * N + y;
 #pragma unroll
     for (int j = 0; j < Q; ++j) {  // x = j (+Q) and H + j (+Q)
-      const double v1 = vcol[j * N], v2 = vcol[(H + j) * N];
-      b[j] = mk(b[j].x * v1, b[j].y * v1);
-      b[Q + j] = mk(b[Q + j].x * v2, b[Q + j].y * v2);
+      const R v1 = (R)vcol[j * N], v2 = (R)vcol[(H + j) * N];
+      b[j] = CplxT<C>::make(b[j].x * v1, b[j].y * v1);
+      b[Q + j] = CplxT<C>::make(b[Q + j].x * v2, b[Q + j].y * v2);
     }
     split_dif_sel<N, -1>(b, odd);
 #pragma unroll
@@ -662,7 +677,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
   __syncthreads();
   if (yrow) {  // Y forward of row a into the sphere columns
     const int v = v0 + yp;
-    const double2 *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
+    const C *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
       b[j] = Pr[j];
@@ -670,7 +685,7 @@ __device__ inline void plane_split_body(const PwDev &d, const double2 *__restric
     }
     split_dif_sel<N, -1>(b, odd);
     const int32_t *cc = d.col_cnt + (int64_t)(group ? group[v] : 0) * nslot + a * nc2 + M2;
-    double2 *Sz = S + ((int64_t)v * N + z) * nslot + a * nc2 + M2;
+    C *Sz = S + ((int64_t)v * N * kSlotScale<C> + z) * nslot + a * nc2 + M2;
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
@@ -1117,12 +1132,13 @@ template <int N, typename F> inline void with_band(bool enable, int m, F &&f) {
 // on it is uniform within a block, so the barriers of either part are safe.
 // SPLIT: the z inverse transforms split over lane pairs (32 columns per
 // 64-lane group), else one lane per column.
-template <int N, bool SPLIT, int MB>
+template <int N, bool SPLIT, int MB, typename C = double2, bool RL = false>
 __global__ __launch_bounds__(256) void k_pw_becp_zinv(
     PwDev d, const double2 *__restrict__ in, int64_t sin, const int32_t *__restrict__ group,
     const int32_t *__restrict__ active, const int32_t *__restrict__ gvec,
     const int32_t *__restrict__ gcnt, int maxper, int tiles, int nchunk, int ngroup,
-    double2 *__restrict__ part, double2 *__restrict__ T, int nvec) {
+    double2 *__restrict__ part, C *__restrict__ T, int nvec,
+    const int32_t *__restrict__ relax = nullptr, int want = 0) {
   const int nbecp = nchunk * ngroup * tiles, b = blockIdx.x;
   if (b < nbecp) {
     becp_body(d, in, sin, active, gvec, gcnt, maxper, tiles, nchunk, part, b % nchunk,
@@ -1135,11 +1151,11 @@ __global__ __launch_bounds__(256) void k_pw_becp_zinv(
   if (zb >= gx * nvec) return;
   const int lane = (int)(threadIdx.x & 63);
   if constexpr (SPLIT)
-    zinv_split_body<N, MB>(d, in, sin, group, active, T, zb / gx, (zb % gx) * 32 + (lane >> 1),
-                           (lane & 1) != 0);
+    zinv_split_body<N, MB, C, RL>(d, in, sin, group, active, T, zb / gx,
+                                  (zb % gx) * 32 + (lane >> 1), (lane & 1) != 0, relax, want);
   else
-    zinv_body<N, double2>(d, in, sin, group, active, T, nullptr, 0, zb / gx,
-                          (zb % gx) * 64 + lane);
+    zinv_body<N, C>(d, in, sin, group, active, T, RL ? relax : nullptr, want, zb / gx,
+                    (zb % gx) * 64 + lane);
 }
 
 // PAIR: two vectors per plane block (plane2_body), else one (plane_body)
@@ -1167,13 +1183,13 @@ __global__ __launch_bounds__(64) void k_pw_reduce_plane(
 // As k_pw_reduce_plane with the split plane transforms (blocks of W
 // wavefronts, PL vectors per plane block): a reduction block covers W of the
 // 64-output ranges.  MB as in plane_split_body.
-template <int N, int MB, int PL, int W>
+template <int N, int MB, int PL, int W, typename C = double2, bool RL = false>
 __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
     PwDev d, const int32_t *__restrict__ group, const int32_t *__restrict__ active,
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
     int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
-    double2 *__restrict__ becp, const double2 *__restrict__ T, double2 *__restrict__ S,
-    int nvec) {
+    double2 *__restrict__ becp, const C *__restrict__ T, C *__restrict__ S, int nvec,
+    const int32_t *__restrict__ relax = nullptr, int want = 0) {
   extern __shared__ double2 smem_d[];
   const int nred_w = (nred_x + W - 1) / W, nred = nred_w * ngroup * tiles, b = blockIdx.x;
   if (b < nred) {
@@ -1184,7 +1200,8 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
     return;
   }
   const int pb = b - nred;
-  plane_split_body<N, MB, PL, W>(d, T, group, active, S, pb % N, PL * (pb / N), nvec, smem_d);
+  plane_split_body<N, MB, PL, W, C, RL>(d, T, group, active, S, pb % N, PL * (pb / N), nvec,
+                                        reinterpret_cast<C *>(smem_d), relax, want);
 }
 
 // planes per block and wavefronts per block of the split plane kernel.
@@ -1396,8 +1413,13 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
   const double2 *locv = nullptr;
   const int nchunk = (int)((ld + kChunk - 1) / kChunk);
   const int nred_x = (nproj_ * kVT + 63) / 64;
-  const bool fused = fuse_stages_ && fft_mode_ == 0 && cubic &&
-                     (n1_ == 24 || n1_ == 32 || n1_ == 36 || n1_ == 48 || n1_ == 64);
+  // FP32 (fft_mode_ 1, or 2 per vector) takes the fused split kernels where
+  // the split plane kernel applies, the separate ones otherwise
+  const bool split_plane_ok = split_planes_ && n1_ <= 48 && nc1_ <= 32 &&
+                              sizeof(double2) * (size_t)nc1_ * (n1_ + 1) <= 64 * 1024;
+  const bool fused = fuse_stages_ && cubic &&
+                     (n1_ == 24 || n1_ == 32 || n1_ == 36 || n1_ == 48 || n1_ == 64) &&
+                     (fft_mode_ == 0 || split_plane_ok);
   if (nproj_ > 0 && !fused) {
     // projector contraction becp = D proj^H x (a concurrent stream for it was
     // measured: no gain, the plane kernel already fills the device)
@@ -1415,6 +1437,20 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
       // no projectors: the part counts are zero and only the transforms run
       const int tiles = nproj_ > 0 ? wk.plan_tiles : 0;
       const int64_t nbecp = (int64_t)nchunk * ngroup_ * tiles;
+      // fft_mode_: 0 FP64, 1 FP32, 2 per vector from the relax flags.  With
+      // both precisions the FP64 instances carry the projector sums and the
+      // FP32 ones (tiles 0) the transforms of the relaxed vectors only.  The
+      // FP64 instances read the flags only in mode 2 (template RL), so mode 0
+      // runs the code it runs without an FP32 path.
+      const int32_t *rl = fft_mode_ == 2 ? relax_ : nullptr;
+      auto with_rl = [&](auto &&f) {
+        if (rl) f(std::true_type{});
+        else f(std::false_type{});
+      };
+      const bool run64 = fft_mode_ != 1, run32 = fft_mode_ != 0;
+      const int tiles32 = run64 ? 0 : tiles;
+      const int64_t nbecp32 = (int64_t)nchunk * ngroup_ * tiles32;
+      float2 *T32 = reinterpret_cast<float2 *>(wk.T), *S32 = reinterpret_cast<float2 *>(wk.S);
       // z transforms split over lane pairs: 32 columns per 64 lanes
       const bool zsplit = split_planes_;
       const int64_t gxz = zsplit ? (nslot_ + 31) / 32 : (int64_t)gcol.x;
@@ -1426,10 +1462,18 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
         KScope ks("pw_becp_zinv", st, 2);
         if (zsplit)
           with_band<NB>(band_planes_, M3_, [&](auto mb) {
-            k_pw_becp_zinv<NB, true, decltype(mb)::value>
-                <<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
-                    d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max,
-                    tiles, nchunk, ngroup_, wk.part, wk.T, nvec);
+            if (run64)
+              with_rl([&](auto rlt) {
+                k_pw_becp_zinv<NB, true, decltype(mb)::value, double2, decltype(rlt)::value>
+                    <<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+                        d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt,
+                        wk.plan_max, tiles, nchunk, ngroup_, wk.part, wk.T, nvec, rl, 0);
+              });
+            if (run32)
+              k_pw_becp_zinv<NB, true, decltype(mb)::value, float2, true>
+                  <<<(unsigned)(nbecp32 + nzinv), 256, 0, st>>>(
+                      d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max,
+                      tiles32, nchunk, ngroup_, wk.part, T32, nvec, rl, 1);
           });
         else
           k_pw_becp_zinv<NB, false, -1><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
@@ -1447,23 +1491,38 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
           // split transforms over lane pairs (LDS: P of the block's planes only)
           constexpr int PL = SplitShape<NB>::PL, W = SplitShape<NB>::W;
           const size_t lds_split = sizeof(double2) * PL * (size_t)nc1_ * (NB + 1);
-          auto launch_split = [&](auto shape, auto mb) {
+          auto launch_split = [&](auto shape, auto mb, auto rlt) {
             constexpr int pl = decltype(shape)::value / 8, w = decltype(shape)::value % 8;
             const int64_t nred_s = (int64_t)((nred_x + w - 1) / w) * ngroup_ * tiles;
-            k_pw_reduce_split<NB, decltype(mb)::value, pl, w>
+            k_pw_reduce_split<NB, decltype(mb)::value, pl, w, double2, decltype(rlt)::value>
                 <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + pl - 1) / pl)), 64 * w,
                    sizeof(double2) * pl * (size_t)nc1_ * (NB + 1), st>>>(
                     d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk,
-                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec);
+                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec, rl, 0);
+          };
+          // the relaxed vectors' FP32 plane transforms, one plane per wavefront
+          auto launch_split32 = [&](auto mb) {
+            const int64_t nred_s = (int64_t)nred_x * ngroup_ * tiles32;
+            k_pw_reduce_split<NB, decltype(mb)::value, 1, 1, float2, true>
+                <<<(unsigned)(nred_s + (int64_t)NB * nvec), 64,
+                   sizeof(float2) * (size_t)nc1_ * (NB + 1), st>>>(
+                    d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles32, nchunk,
+                    ngroup_, nred_x, wk.part, wk.becp, T32, S32, nvec, rl, 1);
           };
           const bool band = band_planes_ && M1_ == M2_;
           if (split_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
             // compile-time sphere half-widths for the usual ones of this box;
             // on the 48 box the autotuner also tries one plane on two
             // wavefronts (the X stage in one round) and two planes on two
-            // for this size class of the batch
+            // for this size class of the batch (mode 2: the default shape)
             with_band<NB>(band, M1_, [&](auto mb) {
               using I11 = std::integral_constant<int, PL * 8 + W>;
+              constexpr std::false_type no_rl{};
+              if (!run64) return;
+              if (rl) {
+                launch_split(I11{}, mb, std::true_type{});
+                return;
+              }
               if constexpr (NB == 48) {
                 const size_t row = sizeof(double2) * (size_t)nc1_ * (NB + 1);
                 const int ncand = (nc1_ <= 32 && 2 * row <= 64 * 1024) ? 3 : 1;
@@ -1471,14 +1530,16 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                                         std::to_string((int)decltype(mb)::value) + "/n" +
                                         std::to_string(size_class(nvec));
                 autotune(key, ncand, st, [&](int c) {
-                  if (c == 0) launch_split(I11{}, mb);
-                  else if (c == 1) launch_split(std::integral_constant<int, 1 * 8 + 2>{}, mb);
-                  else launch_split(std::integral_constant<int, 2 * 8 + 2>{}, mb);
+                  if (c == 0) launch_split(I11{}, mb, no_rl);
+                  else if (c == 1)
+                    launch_split(std::integral_constant<int, 1 * 8 + 2>{}, mb, no_rl);
+                  else launch_split(std::integral_constant<int, 2 * 8 + 2>{}, mb, no_rl);
                 });
               } else {
-                launch_split(I11{}, mb);
+                launch_split(I11{}, mb, no_rl);
               }
             });
+            if (run32) with_band<NB>(band, M1_, [&](auto mb) { launch_split32(mb); });
             paired = true;
           } else if (pair_planes_ && 2 * nc1_ <= 64 && 2 * lds_fft <= 40 * 1024) {
             k_pw_reduce_plane<NB, true>
@@ -1498,8 +1559,16 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
         KScope ks("pw_zfwd", st, 2);
         if (zsplit)
           with_band<NB>(band_planes_, M3_, [&](auto mb) {
-            k_pw_zfwd_split<NB, decltype(mb)::value>
-                <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, wk.S, d_group, d_active, wk.loc);
+            if (run64)
+              with_rl([&](auto rlt) {
+                k_pw_zfwd_split<NB, decltype(mb)::value, double2, decltype(rlt)::value>
+                    <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, wk.S, d_group, d_active,
+                                                               wk.loc, rl, 0);
+              });
+            if (run32)
+              k_pw_zfwd_split<NB, decltype(mb)::value, float2, true>
+                  <<<dim3((unsigned)gxz, nvec), 64, 0, st>>>(d, S32, d_group, d_active, wk.loc,
+                                                             rl, 1);
           });
         else
           k_pw_zfwd<NB><<<gcol, 64, 0, st>>>(d, wk.S, d_group, d_active, wk.loc, nullptr, 0);

@@ -82,27 +82,37 @@ def test_happly_48cubed_matches_oracle(m, band, monkeypatch):
 
 @pytest.fixture(scope="module")
 def solved_c2():
-    """All 8 systems of the 25 Ha cell solved on the device at 1e-12."""
+    """All 8 systems of the 25 Ha cell solved on the device at 1e-12 and at
+    1e-10."""
     w = _cell(8)
     op = operators.PwOperator.from_workload(w)
-    print("48^3 solve at 1e-12: 8 systems x 32 shifts", flush=True)
+    print("48^3 solves at 1e-12 and 1e-10: 8 systems x 32 shifts", flush=True)
     x, ierr, iters = op.solve_batched(w.rhs, w.sigma, group=w.group, threshold=1e-12)
-    return w, x, ierr, iters
+    _, ierr10, iters10 = op.solve_batched(w.rhs, w.sigma, group=w.group, threshold=1e-10)
+    return w, x, ierr, iters, ierr10, iters10
 
 
 @pytest.mark.parametrize("s", [0, 3, 5])
 def test_solve_48cubed_matches_oracle(solved_c2, s):
     """Bands 0 and 3 of q 0 and band 1 of q 1: every one of the 32 shifted
-    solutions within 1e-10 relative of the oracle's FP64 multishift solve."""
-    w, x, ierr, iters = solved_c2
-    assert ierr[s] == 0
+    solutions of the 1e-12 solve within 1e-10 relative of the oracle's FP64
+    multishift solve, and the exit iteration of the 1e-10 solve within
+    ITER_JITTER of the oracle's.  (Near 1e-12 the last iterations stagnate at
+    the attainable accuracy and the exit iteration follows the rounding: two
+    FP64 builds of the H-apply 1 ulp apart exit at 26 and 28 on system 5,
+    the oracle at 25; at 1e-10 the recurrences still decide it.)"""
+    w, x, ierr, iters, ierr10, iters10 = solved_c2
+    assert ierr[s] == 0 and ierr10[s] == 0
     g = int(w.group[s])
     n = int(w.npw[g])
     print(f"oracle bicgstab 48^3 system {s}", flush=True)
     xo, eo, io = orc.bicgstab(orc.PwOp(w, g), w.rhs[s, :n], w.sigma[s], lmax=4, threshold=1e-12)
-    assert eo == 0 and abs(io - iters[s]) <= ITER_JITTER, (io, iters[s])
+    assert eo == 0
     rel = np.linalg.norm(x[s, :, :n] - xo, axis=1) / np.linalg.norm(xo, axis=1)
     assert rel.max() <= 1e-10, rel.max()
+    _, eo10, io10 = orc.bicgstab(orc.PwOp(w, g), w.rhs[s, :n], w.sigma[s], lmax=4,
+                                 threshold=1e-10)
+    assert eo10 == 0 and abs(io10 - iters10[s]) <= ITER_JITTER, (io10, iters10[s])
 
 
 def test_streamed_solve_and_residual_check_against_oracle():
