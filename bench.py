@@ -753,7 +753,7 @@ def main():
     solver_q = solver
     if args.stream and slots != nsys:  # a streaming solver with its own slot count
         cfg_q = _lib.sgw_batch_config(slots, nshift, ld, args.lmax, 10000, args.threshold,
-                                      args.shift_test, 0.0)
+                                      args.shift_test, relax)
         solver_q = L.sgw_batch_solver_create(op.handle, ctypes.byref(cfg_q))
         if not solver_q:
             raise RuntimeError(L.sgw_last_error().decode())
@@ -776,8 +776,6 @@ def main():
     # (each batch is the same 8192 solves; every solve is completed and
     # checked below)
     nq = nsys * max(args.steps, args.warmup, 1)
-    if args.stream and relax > 0:
-        raise SystemExit("bench.py: --stream runs FP64 only (use --stream 0 with --precision mixed)")
     if args.stream:
         reps = nq // nsys
         d_rhs_q = d_rhs.repeat(reps, 1).contiguous()

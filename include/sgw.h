@@ -226,9 +226,11 @@ int sgw_batch_solver_run(sgw_batch_solver *solver, const int32_t *d_group,
  * runs exactly the iterations of sgw_batch_solver_run (a system's iterates do
  * not depend on the other slots).  h_group[nq] host; d_rhs[nq][ld],
  * d_sigma/h_sigma[nq][nshift], d_x[nq][nshift][ld]; h_ierr/h_iters[nq].
- * Device-resident operators; relax_threshold is ignored (FP64).  A system
- * that reaches max_iter stops one iteration later than in the lock-step
- * solve (ierr 1 either way). */
+ * Device-resident operators.  With relax_threshold > 0 (plane-wave
+ * operators) a slot's local-potential FFTs switch to FP32 once its residual
+ * estimate is below it, and a refilled slot starts again in FP64.  A system
+ * that reaches max_iter stops in the same iteration as in the lock-step
+ * solve (ierr 1). */
 int sgw_batch_solver_run_stream(sgw_batch_solver *solver, int32_t nq, const int32_t *h_group,
                                 const sgw_complex *d_rhs, const sgw_complex *d_sigma,
                                 const sgw_complex *h_sigma, sgw_complex *d_x, int32_t *h_ierr,
