@@ -128,6 +128,13 @@ class PwOperator : public Operator {
   DevBuf<int32_t> d_npw_, col_start_, col_cnt_, col_lo_, sidx_, sm3_, slot_of_, m3_of_;
   DevBuf<double2> proj_, tw_, S_, T_, loc_;
   bool columns_contiguous_ = true;
+  // column layouts of the split z and plane transforms' work arrays: the
+  // packed one (per group, each row of the column grid from its first to its
+  // last column holding plane waves; ncs_ columns per plane) and the full
+  // grid (nslot_ columns, the layout of every other kernel)
+  int ncs_ = 0;
+  DevBuf<int32_t> pk_slot_, pk_row_start_, pk_row_lo_, pk_row_cnt_;
+  DevBuf<int32_t> gr_slot_, gr_row_start_, gr_row_lo_, gr_row_cnt_;
 };
 
 }  // namespace sgw

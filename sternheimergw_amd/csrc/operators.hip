@@ -77,6 +77,15 @@ struct PwDev {
   const int32_t *slot_of;      // [ngroup][ld]
   const int32_t *m3_of;        // [ngroup][ld] (reduced mod n3)
   const int32_t *col_lo;       // [ngroup][nslot] smallest m3 of the column
+  // column layout of the split z / plane transforms' T and S: ncs columns per
+  // plane; packed column j of group g is grid slot col_slot[g][j] (-1: none);
+  // row a of the grid holds columns m2 = row_lo .. row_lo + row_cnt - 1 at
+  // j = row_start[g][a] + m2 - row_lo (row_start[g][nc1]: the group's count)
+  int ncs;
+  const int32_t *col_slot;     // [ngroup][ncs]
+  const int32_t *row_start;    // [ngroup][nc1 + 1]
+  const int32_t *row_lo;       // [ngroup][nc1]
+  const int32_t *row_cnt;      // [ngroup][nc1]
   const double *kin;           // [ngroup][ld]
   const double2 *proj;         // [ngroup][nproj][ld]
   const double *dcoef;         // [ngroup][nproj]
@@ -301,32 +310,36 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
   }
 }
 
-// FP64 z transforms with each column split over a lane pair (splitfft.h):
-// lane 2c + odd of the pair of column c holds N/2 of its values, so these
-// need about half the registers of k_pw_zinv / k_pw_zfwd.  Both lanes of a
-// pair take the same early exits.  MB >= 0: the sphere's z half-width M3 as
-// a compile-time constant, so values outside |m3| <= MB are literal zeros
-// (inverse) or never stored (forward); -1: no such bound.
+// z transforms with each column split over a lane pair (splitfft.h): lane
+// 2j + odd of the pair of column j holds N/2 of its values, so these need
+// about half the registers of k_pw_zinv / k_pw_zfwd.  Columns are counted in
+// the layout of d.col_slot (T[v][z][j], d.ncs columns per plane).  Both lanes
+// of a pair take the same early exits.  MB >= 0: the sphere's z half-width
+// M3 as a compile-time constant, so values outside |m3| <= MB are literal
+// zeros (inverse) or never stored (forward); -1: no such bound.
 template <int N, int MB, typename C = double2, bool RL = false>
 __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict__ in,
                                        int64_t sin, const int32_t *__restrict__ group,
                                        const int32_t *__restrict__ active, C *__restrict__ T,
-                                       int v, int c, bool odd,
+                                       int v, int jc, bool odd,
                                        const int32_t *__restrict__ relax = nullptr,
                                        int want = 0) {
   constexpr int H = N / 2, Q = N / 4;
   if (active && !active[v]) return;
   if (precision_skip<RL>(relax, want, v)) return;
-  if (c >= d.nslot) return;
+  const int ncs = d.ncs;
+  if (jc >= ncs) return;
   const int g = group ? group[v] : 0;
+  const int c = d.col_slot[(int64_t)g * ncs + jc];
+  if (c < 0) return;  // past this group's columns
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
-  C *Tv = T + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
+  C *Tv = T + (int64_t)v * N * ncs * kSlotScale<C> + jc;
   const int zo = odd ? Q : 0;  // this lane's z: zo + j and H + zo + j
   if (cnt == 0) {
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
-      Tv[(int64_t)(zo + j) * d.nslot] = czero<C>();
-      Tv[(int64_t)(H + zo + j) * d.nslot] = czero<C>();
+      Tv[(int64_t)(zo + j) * ncs] = czero<C>();
+      Tv[(int64_t)(H + zo + j) * ncs] = czero<C>();
     }
     return;
   }
@@ -343,8 +356,8 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
   split_dit<N, +1>(b, odd);
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
-    Tv[(int64_t)(zo + j) * d.nslot] = b[j];
-    Tv[(int64_t)(H + zo + j) * d.nslot] = b[Q + j];
+    Tv[(int64_t)(zo + j) * ncs] = b[j];
+    Tv[(int64_t)(H + zo + j) * ncs] = b[Q + j];
   }
 }
 
@@ -356,23 +369,26 @@ __global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const C *__restri
                                                       const int32_t *__restrict__ relax = nullptr,
                                                       int want = 0) {
   constexpr int H = N / 2, Q = N / 4;
-  const int v = blockIdx.y, c = blockIdx.x * 32 + (int)(threadIdx.x >> 1);
+  const int v = blockIdx.y, jc = blockIdx.x * 32 + (int)(threadIdx.x >> 1);
   const bool odd = (threadIdx.x & 1) != 0;
   if (active && !active[v]) return;
   if (precision_skip<RL>(relax, want, v)) return;
-  if (c >= d.nslot) return;
+  const int ncs = d.ncs;
+  if (jc >= ncs) return;
   const int g = group ? group[v] : 0;
+  const int c = d.col_slot[(int64_t)g * ncs + jc];
+  if (c < 0) return;  // past this group's columns
   const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
   if (cnt == 0) return;
   const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
-  const C *Sv = S + (int64_t)v * N * d.nslot * kSlotScale<C> + c;
+  const C *Sv = S + (int64_t)v * N * ncs * kSlotScale<C> + jc;
   const int zo = odd ? Q : 0;
   C b[H];
 #pragma unroll
   for (int j = 0; j < Q; ++j) {
-    b[j] = Sv[(int64_t)(zo + j) * d.nslot];
-    b[Q + j] = Sv[(int64_t)(H + zo + j) * d.nslot];
+    b[j] = Sv[(int64_t)(zo + j) * ncs];
+    b[Q + j] = Sv[(int64_t)(H + zo + j) * ncs];
   }
   split_dif<N, -1>(b, odd);
   double2 *lv = loc + (int64_t)v * d.ld;
@@ -572,18 +588,20 @@ __device__ inline void plane2_body(const PwDev &d, const double2 *__restrict__ T
   }
 }
 
-// FP64 plane transforms of PL vectors, planes (z, v0 + p) for p < PL, by a
+// Plane transforms of PL vectors, planes (z, v0 + p) for p < PL, by a
 // block of 64 W lanes whose transforms are split over lane pairs
 // (splitfft.h): every lane holds N/2 values, so the block needs about half
 // the registers per lane of plane2_body and each SIMD holds two wavefronts.
+// T and S are in the column layout of d.row_* ([v][z][j], d.ncs columns per
+// plane; row a of the grid holds its columns m2 = lo .. hi - 1).
 //   Y  lane pair r < PL nc1: row a of plane yp (r = yp nc1 + a), m2 -> y;
 //   X  the PL N (plane, column) tasks in rounds of 32 W lane pairs: column y
 //      of plane p, m1 -> x, times V(x, y, z), x -> m1 (nc1 frequencies kept);
-//   Y' as Y, y -> m2, into the sphere columns S[v][z][c].
+//   Y' as Y, y -> m2, into the row's columns of S.
 // With PL N a multiple of 32 W (4 planes of 48 on 3 wavefronts) every lane
 // pair works in every X round.  LDS holds P[PL][nc1][N + 1] (rows padded by
 // one value, so that the lanes of the Y stages, one row each, fall on
-// different banks); the column values T[PL][nslot] are staged into its start
+// different banks); the column values T[PL][ncs] are staged into its start
 // and read into registers before the Y stage overwrites them.
 // MB >= 0: the sphere's half-widths M1 = M2 = MB are compile-time constants.
 // Every transform input outside |m| <= MB is then a literal zero and every
@@ -601,8 +619,8 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
   using R = typename CplxT<C>::S;
   constexpr int H = N / 2, Q = N / 4, NP = 32 * W;  // NP lane pairs
   constexpr int PN = N + 1;  // row stride of P: rows of N values would share LDS banks
-  const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1, nc2 = MB >= 0 ? 2 * MB + 1 : d.nc2;
-  const int nslot = d.nslot, M2 = MB >= 0 ? MB : d.M2;
+  const int nc1 = MB >= 0 ? 2 * MB + 1 : d.nc1;
+  const int ncs = d.ncs, M2 = MB >= 0 ? MB : d.M2;
   const int t = threadIdx.x, r = t >> 1;
   const bool odd = (t & 1) != 0;
   bool live[PL], any = false;
@@ -613,21 +631,26 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
     any |= live[p];
   }
   if (!any) return;
-  C *P = smem, *T = smem;  // T[PL][nslot] aliases the start of P[PL][nc1][PN]
+  C *P = smem, *T = smem;  // T[PL][ncs] aliases the start of P[PL][nc1][PN]
 #pragma unroll
   for (int p = 0; p < PL; ++p)  // plane p staged by wavefront p mod W
-    if ((t >> 6) == p % W && live[p])
-      stage_plane(Tin + ((int64_t)(v0 + p) * N * kSlotScale<C> + z) * nslot, T + p * nslot, nslot);
+    if ((t >> 6) == p % W && live[p]) {
+      const int gp = group ? group[v0 + p] : 0;
+      stage_plane(Tin + ((int64_t)(v0 + p) * N * kSlotScale<C> + z) * ncs, T + p * ncs,
+                  d.row_start[(int64_t)gp * (nc1 + 1) + nc1]);
+    }
   __syncthreads();
   const int yp = r / nc1, a = r - yp * nc1;
   const bool yrow = yp < PL && live[yp];
   C b[H];
-  if (yrow) {
-    const C *Tp = T + yp * nslot + a * nc2 + M2;
+  if (yrow) {  // row a: columns m2 = lo .. hi - 1 of the group's layout
+    const int64_t g = group ? group[v0 + yp] : 0;
+    const int lo = d.row_lo[g * nc1 + a], hi = lo + d.row_cnt[g * nc1 + a];
+    const C *Tp = T + yp * ncs + d.row_start[g * (nc1 + 1) + a] - lo;
 #pragma unroll
     for (int n = 0; n < H; ++n) {
       const int i = 2 * n + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      b[n] = (m >= -M2 && m <= M2) ? Tp[m] : czero<C>();
+      b[n] = (m >= -M2 && m <= M2 && m >= lo && m < hi) ? Tp[m] : czero<C>();
     }
   }
   __syncthreads();  // every column value is in registers before P overwrites T
@@ -684,12 +707,13 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
       b[Q + j] = Pr[H + j];
     }
     split_dif_sel<N, -1>(b, odd);
-    const int32_t *cc = d.col_cnt + (int64_t)(group ? group[v] : 0) * nslot + a * nc2 + M2;
-    C *Sz = S + ((int64_t)v * N * kSlotScale<C> + z) * nslot + a * nc2 + M2;
+    const int64_t g = group ? group[v] : 0;
+    const int lo = d.row_lo[g * nc1 + a], hi = lo + d.row_cnt[g * nc1 + a];
+    C *Sz = S + ((int64_t)v * N * kSlotScale<C> + z) * ncs + d.row_start[g * (nc1 + 1) + a] - lo;
 #pragma unroll
     for (int k = 0; k < H; ++k) {
       const int i = 2 * k + (odd ? 1 : 0), m = i < N / 2 ? i : i - N;
-      if (m >= -M2 && m <= M2 && cc[m] > 0) Sz[m] = b[k];
+      if (m >= -M2 && m <= M2 && m >= lo && m < hi) Sz[m] = b[k];
     }
   }
 }
@@ -1146,7 +1170,7 @@ __global__ __launch_bounds__(256) void k_pw_becp_zinv(
     return;
   }
   constexpr int kCols = SPLIT ? 32 : 64;  // columns per 64-lane group
-  const int gx = (d.nslot + kCols - 1) / kCols;
+  const int gx = ((SPLIT ? d.ncs : d.nslot) + kCols - 1) / kCols;
   const int zb = (b - nbecp) * 4 + (int)(threadIdx.x >> 6);
   if (zb >= gx * nvec) return;
   const int lane = (int)(threadIdx.x & 63);
@@ -1335,6 +1359,57 @@ PwOperator::PwOperator(const PwDesc &desc) {
       sm3[(size_t)g * ld + j] = m3_of[(size_t)g * ld + i];
     }
   }
+  {
+    // packed column layout: per group, row a of the grid from its first to
+    // its last column with plane waves (a row of a sphere's columns is one
+    // run, so the packed planes hold exactly the sphere's columns; a gap
+    // would be kept, as zeros)
+    const int nr = nc1_ + 1;
+    std::vector<int32_t> rs((size_t)ngroup_ * nr, 0), rl((size_t)ngroup_ * nc1_, 0),
+        rc((size_t)ngroup_ * nc1_, 0);
+    int ncs = 1;
+    for (int g = 0; g < ngroup_; ++g) {
+      int j = 0;
+      for (int a = 0; a < nc1_; ++a) {
+        int lo = nc2_, hi = -1;
+        for (int b = 0; b < nc2_; ++b)
+          if (col_cnt[(size_t)g * nslot_ + a * nc2_ + b] > 0) {
+            lo = std::min(lo, b);
+            hi = std::max(hi, b);
+          }
+        rs[(size_t)g * nr + a] = j;
+        if (hi >= lo) {
+          rl[(size_t)g * nc1_ + a] = lo - M2_;
+          rc[(size_t)g * nc1_ + a] = hi - lo + 1;
+          j += hi - lo + 1;
+        }
+      }
+      rs[(size_t)g * nr + nc1_] = j;
+      ncs = std::max(ncs, j);
+    }
+    ncs_ = ncs;
+    std::vector<int32_t> cs((size_t)ngroup_ * ncs_, -1);
+    for (int g = 0; g < ngroup_; ++g)
+      for (int a = 0; a < nc1_; ++a)
+        for (int k = 0; k < rc[(size_t)g * nc1_ + a]; ++k)
+          cs[(size_t)g * ncs_ + rs[(size_t)g * nr + a] + k] =
+              a * nc2_ + rl[(size_t)g * nc1_ + a] + M2_ + k;
+    pk_slot_.alloc(cs.size()); pk_slot_.upload(cs.data(), cs.size());
+    pk_row_start_.alloc(rs.size()); pk_row_start_.upload(rs.data(), rs.size());
+    pk_row_lo_.alloc(rl.size()); pk_row_lo_.upload(rl.data(), rl.size());
+    pk_row_cnt_.alloc(rc.size()); pk_row_cnt_.upload(rc.data(), rc.size());
+    // the full grid in the same terms
+    std::vector<int32_t> gs((size_t)ngroup_ * nslot_), grs((size_t)ngroup_ * nr),
+        grl((size_t)ngroup_ * nc1_, -M2_), grc((size_t)ngroup_ * nc1_, nc2_);
+    for (int g = 0; g < ngroup_; ++g) {
+      for (int c = 0; c < nslot_; ++c) gs[(size_t)g * nslot_ + c] = c;
+      for (int a = 0; a <= nc1_; ++a) grs[(size_t)g * nr + a] = a * nc2_;
+    }
+    gr_slot_.alloc(gs.size()); gr_slot_.upload(gs.data(), gs.size());
+    gr_row_start_.alloc(grs.size()); gr_row_start_.upload(grs.data(), grs.size());
+    gr_row_lo_.alloc(grl.size()); gr_row_lo_.upload(grl.data(), grl.size());
+    gr_row_cnt_.alloc(grc.size()); gr_row_cnt_.upload(grc.data(), grc.size());
+  }
   const int64_t nbox = (int64_t)n1_ * n2_ * n3_;
   vloc_.alloc(nbox); vloc_.upload(desc.vloc, nbox);
   {  // [z][x][y] copy for the x stage of the plane kernels
@@ -1404,6 +1479,8 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
+  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
   d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
@@ -1453,7 +1530,15 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
       float2 *T32 = reinterpret_cast<float2 *>(wk.T), *S32 = reinterpret_cast<float2 *>(wk.S);
       // z transforms split over lane pairs: 32 columns per 64 lanes
       const bool zsplit = split_planes_;
-      const int64_t gxz = zsplit ? (nslot_ + 31) / 32 : (int64_t)gcol.x;
+      // T and S in the packed column layout where the split plane kernel
+      // runs (it and the split z transforms are their only readers and
+      // writers here); the full grid for the other plane kernels
+      const bool packed = zsplit && split_plane_ok;
+      if (packed) {
+        d.ncs = ncs_; d.col_slot = pk_slot_.p;
+        d.row_start = pk_row_start_.p; d.row_lo = pk_row_lo_.p; d.row_cnt = pk_row_cnt_.p;
+      }
+      const int64_t gxz = zsplit ? (d.ncs + 31) / 32 : (int64_t)gcol.x;
       const int64_t nzinv = (gxz * nvec + 3) / 4;
       const int64_t nred = (int64_t)nred_x * ngroup_ * tiles;
       SGW_REQUIRE(nbecp + nzinv < INT32_MAX && nred + (int64_t)NB * nvec < INT32_MAX, SGW_E_ARG,
@@ -1510,7 +1595,10 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                     ngroup_, nred_x, wk.part, wk.becp, T32, S32, nvec, rl, 1);
           };
           const bool band = band_planes_ && M1_ == M2_;
-          if (split_planes_ && PL * nc1_ <= 32 * W && lds_split <= 64 * 1024) {
+          static_assert(PL == 1 && W == 1, "the packed-layout test assumes one plane per wave");
+          if (packed) {  // split_planes_, nc1_ <= 32 and lds_split <= 64 KB
+            SGW_REQUIRE(lds_split <= 64 * 1024 && PL * nc1_ <= 32 * W, SGW_E_STATE,
+                        "pw operator: split plane kernel does not fit");
             // compile-time sphere half-widths for the usual ones of this box;
             // on the 48 box the autotuner also tries one plane on two
             // wavefronts (the X stage in one round) and two planes on two
@@ -1674,6 +1762,8 @@ void PwOperator::drho_accumulate(hipStream_t st, int group, int nbnd, int nshift
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
+  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
   d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
@@ -1724,6 +1814,8 @@ void PwOperator::grad_accumulate(hipStream_t st, int np, const double2 *x, const
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
+  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
   d.inv_nbox = 1.0 / ((double)n1_ * n2_ * n3_);
