@@ -533,10 +533,10 @@ int sgw_kernel_timing(int enable);
 int sgw_kernel_timing_query(const char *name, double *total_ms, int64_t *launches);
 
 /* Launch-shape autotuning.  No counterpart in the reference.  The plane
- * transforms on the 48 box (one plane per wavefront, one plane on two
- * wavefronts, two planes on two) and the final assembly of an application
- * (16, 32 or 64 plane waves per block) have several launch shapes with the
- * same results; the first application of each size class (batch size
+ * transforms on the 48 box (three planes on two wavefronts, one plane per
+ * wavefront, one plane on two, two planes on two) and the final assembly of
+ * an application (16, 32 or 64 plane waves per block) have several launch
+ * shapes with the same results; the first application of each size class (batch size
  * rounded up to a power of two) times every shape once and keeps the fastest.
  * SGW_AUTOTUNE=0 in the environment keeps the first shape.  Returns the
  * candidate index chosen for `key` ("plane48/M<m>/b<band>/n<class>",

@@ -55,6 +55,7 @@ struct PwWork {
   double2 *T, *S, *loc, *part, *becp;
   const int32_t *plan_vec, *plan_cnt;
   int plan_max, plan_tiles;
+  int32_t *vlist;  // [nvec]: the running vectors in order, then -1
 };
 
 class PwOperator : public Operator {
@@ -127,6 +128,7 @@ class PwOperator : public Operator {
   DevBuf<double> vloc_, vloc_zxy_, kin_, dcoef_;
   DevBuf<int32_t> d_npw_, col_start_, col_cnt_, col_lo_, sidx_, sm3_, slot_of_, m3_of_;
   DevBuf<double2> proj_, tw_, S_, T_, loc_;
+  DevBuf<int32_t> vlist_;
   bool columns_contiguous_ = true;
   // column layouts of the split z and plane transforms' work arrays: the
   // packed one (per group, each row of the column grid from its first to its

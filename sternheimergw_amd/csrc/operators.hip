@@ -617,7 +617,8 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
                                         const int32_t *__restrict__ active,
                                         C *__restrict__ S, int z, int v0, int nvec, C *smem,
                                         const int32_t *__restrict__ relax = nullptr,
-                                        int want = 0) {
+                                        int want = 0,
+                                        const int32_t *__restrict__ vlist = nullptr) {
   using R = typename CplxT<C>::S;
   constexpr int H = N / 2, Q = N / 4, NP = 32 * W;  // NP lane pairs
   constexpr int PN = N + 1;  // row stride of P: rows of N values would share LDS banks
@@ -625,11 +626,16 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
   const int ncs = d.ncs, M2 = MB >= 0 ? MB : d.M2;
   const int t = threadIdx.x, r = t >> 1;
   const bool odd = (t & 1) != 0;
+  // plane p of the block belongs to vector vlist[v0 + p] (the running
+  // vectors in order, then -1) or, without a list, to vector v0 + p
+  int vv[PL];
   bool live[PL], any = false;
 #pragma unroll
   for (int p = 0; p < PL; ++p) {
-    live[p] = v0 + p < nvec && (!active || active[v0 + p]) &&
-              !precision_skip<RL>(relax, want, v0 + p);
+    const int i = v0 + p;
+    vv[p] = i < nvec ? (vlist ? vlist[i] : i) : -1;
+    live[p] = vv[p] >= 0 && (!active || active[vv[p]]) &&
+              !precision_skip<RL>(relax, want, vv[p]);
     any |= live[p];
   }
   if (!any) return;
@@ -637,16 +643,17 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
 #pragma unroll
   for (int p = 0; p < PL; ++p)  // plane p staged by wavefront p mod W
     if ((t >> 6) == p % W && live[p]) {
-      const int gp = group ? group[v0 + p] : 0;
-      stage_plane(Tin + ((int64_t)(v0 + p) * N * kSlotScale<C> + z) * ncs, T + p * ncs,
+      const int gp = group ? group[vv[p]] : 0;
+      stage_plane(Tin + ((int64_t)vv[p] * N * kSlotScale<C> + z) * ncs, T + p * ncs,
                   d.row_start[(int64_t)gp * (nc1 + 1) + nc1]);
     }
   __syncthreads();
   const int yp = r / nc1, a = r - yp * nc1;
   const bool yrow = yp < PL && live[yp];
+  const int vy = yrow ? (vlist ? vlist[v0 + yp] : v0 + yp) : 0;  // this lane pair's vector
   C b[H];
   if (yrow) {  // row a: columns m2 = lo .. hi - 1 of the group's layout
-    const int64_t g = group ? group[v0 + yp] : 0;
+    const int64_t g = group ? group[vy] : 0;
     const int lo = d.row_lo[g * nc1 + a], hi = lo + d.row_cnt[g * nc1 + a];
     const C *Tp = T + yp * ncs + d.row_start[g * (nc1 + 1) + a] - lo;
 #pragma unroll
@@ -701,7 +708,7 @@ __device__ inline void plane_split_body(const PwDev &d, const C *__restrict__ Ti
   }
   __syncthreads();
   if (yrow) {  // Y forward of row a into the sphere columns
-    const int v = v0 + yp;
+    const int v = vy;
     const C *Pr = P + (yp * nc1 + a) * PN + (odd ? Q : 0);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
@@ -1158,14 +1165,47 @@ template <int N, typename F> inline void with_band(bool enable, int m, F &&f) {
 // on it is uniform within a block, so the barriers of either part are safe.
 // SPLIT: the z inverse transforms split over lane pairs (32 columns per
 // 64-lane group), else one lane per column.
+// vlist[0 .. k) = the vectors v < nvec with active[v] != 0 (all of them
+// without flags) in increasing order, vlist[k .. nvec) = -1; one block of
+// 256 lanes
+__device__ inline void vlist_body(int nvec, const int32_t *__restrict__ active,
+                                  int32_t *__restrict__ vlist) {
+  __shared__ int wsum[4];
+  __shared__ int base;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  if (t == 0) base = 0;
+  __syncthreads();
+  for (int v0 = 0; v0 < nvec; v0 += 256) {
+    const int v = v0 + t;
+    const bool on = v < nvec && (!active || active[v] != 0);
+    const unsigned long long m = __ballot(on);
+    if (lane == 0) wsum[w] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int i = 0; i < w; ++i) off += wsum[i];
+    if (on) vlist[off + __popcll(m & ((1ull << lane) - 1))] = v;
+    __syncthreads();  // every lane has read base and wsum
+    if (t == 0) base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  for (int i = base + t; i < nvec; i += 256) vlist[i] = -1;
+}
+
 template <int N, bool SPLIT, int MB, typename C = double2, bool RL = false>
 __global__ __launch_bounds__(256) void k_pw_becp_zinv(
     PwDev d, const double2 *__restrict__ in, int64_t sin, const int32_t *__restrict__ group,
     const int32_t *__restrict__ active, const int32_t *__restrict__ gvec,
     const int32_t *__restrict__ gcnt, int maxper, int tiles, int nchunk, int ngroup,
     double2 *__restrict__ part, C *__restrict__ T, int nvec,
-    const int32_t *__restrict__ relax = nullptr, int want = 0) {
-  const int nbecp = nchunk * ngroup * tiles, b = blockIdx.x;
+    const int32_t *__restrict__ relax = nullptr, int want = 0,
+    int32_t *__restrict__ vlist = nullptr) {
+  // with a list to build, block 0 builds it (first, so that its serial
+  // scan does not trail the launch) and the others shift down by one
+  const int nbecp = nchunk * ngroup * tiles, b = (int)blockIdx.x - (vlist ? 1 : 0);
+  if (b < 0) {
+    vlist_body(nvec, active, vlist);
+    return;
+  }
   if (b < nbecp) {
     becp_body(d, in, sin, active, gvec, gcnt, maxper, tiles, nchunk, part, b % nchunk,
               (b / nchunk) % ngroup, b / (nchunk * ngroup));
@@ -1215,7 +1255,8 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
     const int32_t *__restrict__ gvec, const int32_t *__restrict__ gcnt, int maxper, int tiles,
     int nchunk, int ngroup, int nred_x, const double2 *__restrict__ part,
     double2 *__restrict__ becp, const C *__restrict__ T, C *__restrict__ S, int nvec,
-    const int32_t *__restrict__ relax = nullptr, int want = 0) {
+    const int32_t *__restrict__ relax = nullptr, int want = 0,
+    const int32_t *__restrict__ vlist = nullptr) {
   extern __shared__ double2 smem_d[];
   const int nred_w = (nred_x + W - 1) / W, nred = nred_w * ngroup * tiles, b = blockIdx.x;
   if (b < nred) {
@@ -1227,7 +1268,7 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
   }
   const int pb = b - nred;
   plane_split_body<N, MB, PL, W, C, RL>(d, T, group, active, S, pb % N, PL * (pb / N), nvec,
-                                        reinterpret_cast<C *>(smem_d), relax, want);
+                                        reinterpret_cast<C *>(smem_d), relax, want, vlist);
 }
 
 // planes per block and wavefronts per block of the split plane kernel.
@@ -1450,6 +1491,7 @@ int64_t PwOperator::group_n(int g) const { return npw_.at(g); }
 
 
 void PwOperator::ensure_scratch(int nvec) {
+  if (vlist_.n < (size_t)nvec) vlist_.alloc(nvec);
   size_t need = (size_t)nvec * nslot_ * n3_;
   if (S_.n < need) S_.alloc(need);
   if (T_.n < need) T_.alloc(need);
@@ -1465,7 +1507,8 @@ void PwOperator::apply(hipStream_t st, int nvec, const double2 *in, int64_t in_s
   SGW_REQUIRE(plan_nvec_ == nvec, SGW_E_STATE,
               "pw operator: prepare() was not called for this batch of vectors");
   ensure_scratch(nvec);
-  PwWork w{T_.p, S_.p, loc_.p, part_.p, becp_.p, plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_};
+  PwWork w{T_.p,       S_.p,      loc_.p,    part_.p,      becp_.p,
+           plan_vec_.p, plan_cnt_.p, plan_max_, plan_tiles_, vlist_.p};
   apply_work(st, nvec, in, in_stride, out, out_stride, d_sigma, d_active, d_group, w);
 }
 
@@ -1536,6 +1579,10 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
       // runs (it and the split z transforms are their only readers and
       // writers here); the full grid for the other plane kernels
       const bool packed = zsplit && split_plane_ok;
+      // the split plane kernel's blocks take the running vectors from a list
+      // (built by the z inverse launch), so that a batch with few running
+      // vectors launches no blocks with dead planes among live ones
+      int32_t *vl = (packed && d_active && wk.vlist) ? wk.vlist : nullptr;
       if (packed) {
         d.ncs = ncs_; d.col_slot = pk_slot_.p;
         d.row_start = pk_row_start_.p; d.row_lo = pk_row_lo_.p; d.row_cnt = pk_row_cnt_.p;
@@ -1549,18 +1596,20 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
         KScope ks("pw_becp_zinv", st, 2);
         if (zsplit)
           with_band<NB>(band_planes_, M3_, [&](auto mb) {
+            // the first launch's extra block (block 0) lists the running vectors
+            int32_t *vl64 = run64 ? vl : nullptr, *vl32 = run64 ? nullptr : vl;
             if (run64)
               with_rl([&](auto rlt) {
                 k_pw_becp_zinv<NB, true, decltype(mb)::value, double2, decltype(rlt)::value>
-                    <<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
+                    <<<(unsigned)(nbecp + nzinv + (vl64 ? 1 : 0)), 256, 0, st>>>(
                         d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt,
-                        wk.plan_max, tiles, nchunk, ngroup_, wk.part, wk.T, nvec, rl, 0);
+                        wk.plan_max, tiles, nchunk, ngroup_, wk.part, wk.T, nvec, rl, 0, vl64);
               });
             if (run32)
               k_pw_becp_zinv<NB, true, decltype(mb)::value, float2, true>
-                  <<<(unsigned)(nbecp32 + nzinv), 256, 0, st>>>(
+                  <<<(unsigned)(nbecp32 + nzinv + (vl32 ? 1 : 0)), 256, 0, st>>>(
                       d, in, in_stride, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max,
-                      tiles32, nchunk, ngroup_, wk.part, T32, nvec, rl, 1);
+                      tiles32, nchunk, ngroup_, wk.part, T32, nvec, rl, 1, vl32);
           });
         else
           k_pw_becp_zinv<NB, false, -1><<<(unsigned)(nbecp + nzinv), 256, 0, st>>>(
@@ -1585,7 +1634,7 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                 <<<(unsigned)(nred_s + (int64_t)NB * ((nvec + pl - 1) / pl)), 64 * w,
                    sizeof(double2) * pl * (size_t)nc1_ * (NB + 1), st>>>(
                     d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles, nchunk,
-                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec, rl, 0);
+                    ngroup_, nred_x, wk.part, wk.becp, wk.T, wk.S, nvec, rl, 0, vl);
           };
           // the relaxed vectors' FP32 plane transforms, one plane per wavefront
           auto launch_split32 = [&](auto mb) {
@@ -1594,7 +1643,7 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                 <<<(unsigned)(nred_s + (int64_t)NB * nvec), 64,
                    sizeof(float2) * (size_t)nc1_ * (NB + 1), st>>>(
                     d, d_group, d_active, wk.plan_vec, wk.plan_cnt, wk.plan_max, tiles32, nchunk,
-                    ngroup_, nred_x, wk.part, wk.becp, T32, S32, nvec, rl, 1);
+                    ngroup_, nred_x, wk.part, wk.becp, T32, S32, nvec, rl, 1, vl);
           };
           const bool band = band_planes_ && M1_ == M2_;
           static_assert(PL == 1 && W == 1, "the packed-layout test assumes one plane per wave");
@@ -1602,9 +1651,11 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
             SGW_REQUIRE(lds_split <= 64 * 1024 && PL * nc1_ <= 32 * W, SGW_E_STATE,
                         "pw operator: split plane kernel does not fit");
             // compile-time sphere half-widths for the usual ones of this box;
-            // on the 48 box the autotuner also tries one plane on two
-            // wavefronts (the X stage in one round) and two planes on two
-            // for this size class of the batch (mode 2: the default shape)
+            // on the 48 box the autotuner picks among three planes on two
+            // wavefronts (Y stage on 51 of 64 lane pairs, X stage in 2.25
+            // rounds), one plane on one, one plane on two (the X stage in one
+            // round) and two planes on two for this size class of the batch
+            // (mode 2: one plane on one)
             with_band<NB>(band, M1_, [&](auto mb) {
               using I11 = std::integral_constant<int, PL * 8 + W>;
               constexpr std::false_type no_rl{};
@@ -1615,13 +1666,20 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
               }
               if constexpr (NB == 48) {
                 const size_t row = sizeof(double2) * (size_t)nc1_ * (NB + 1);
-                const int ncand = (nc1_ <= 32 && 2 * row <= 64 * 1024) ? 3 : 1;
+                const bool fit2 = nc1_ <= 32 && 2 * row <= 64 * 1024;
+                const bool fit3 = fit2 && 3 * nc1_ <= 64 && 3 * row <= 64 * 1024;
+                const int ncand = fit2 ? (fit3 ? 4 : 3) : 1;
                 const std::string key = "plane48/M" + std::to_string(M1_) + "/b" +
                                         std::to_string((int)decltype(mb)::value) + "/n" +
                                         std::to_string(size_class(nvec));
+                // shapes (planes, wavefronts): (3, 2) first where it fits (the
+                // fastest measured at 512 and 1024 vectors: 16 % less plane
+                // time than (1, 1)), then (1, 1), (1, 2), (2, 2)
                 autotune(key, ncand, st, [&](int c) {
-                  if (c == 0) launch_split(I11{}, mb, no_rl);
-                  else if (c == 1)
+                  const int k = fit3 ? c : c + 1;
+                  if (k == 0) launch_split(std::integral_constant<int, 3 * 8 + 2>{}, mb, no_rl);
+                  else if (k == 1) launch_split(I11{}, mb, no_rl);
+                  else if (k == 2)
                     launch_split(std::integral_constant<int, 1 * 8 + 2>{}, mb, no_rl);
                   else launch_split(std::integral_constant<int, 2 * 8 + 2>{}, mb, no_rl);
                 });
@@ -1950,7 +2008,8 @@ void PwOperator::apply_noalloc(hipStream_t st, int nvec, const int32_t *d_group,
   const size_t b_becp = align256(sizeof(double2) * (size_t)ngroup_ * tiles * np * kVT);
   const size_t b_box = align256(sizeof(double2) * (size_t)nvec * nslot_ * n3_);
   const size_t b_loc = align256(sizeof(double2) * (size_t)nvec * ld);
-  const size_t total = b_plan + b_part + b_becp + 2 * b_box + b_loc;
+  const size_t b_vlist = align256(sizeof(int32_t) * (size_t)nvec);
+  const size_t total = b_plan + b_part + b_becp + 2 * b_box + b_loc + b_vlist;
   SGW_REQUIRE(total <= (size_t)SGW_NOALLOC_SCRATCH_BYTES, SGW_E_SHAPE,
               "pw operator no-alloc: this batch needs " + std::to_string(total) +
                   " bytes of work space, more than the fixed " +
@@ -1969,6 +2028,8 @@ void PwOperator::apply_noalloc(hipStream_t st, int nvec, const int32_t *d_group,
   w.S = reinterpret_cast<double2 *>(p);
   p += b_box;
   w.loc = reinterpret_cast<double2 *>(p);
+  p += b_loc;
+  w.vlist = reinterpret_cast<int32_t *>(p);
   w.plan_vec = d_plan;
   w.plan_cnt = d_plan + (size_t)ngroup_ * maxper;
   w.plan_max = maxper;

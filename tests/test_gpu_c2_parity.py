@@ -165,7 +165,7 @@ def test_autotuned_shapes_agree_bitwise(monkeypatch):
         x[v, :n] = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     sig = rng.standard_normal(nv) + 1j * rng.standard_normal(nv)
     out = []
-    for c in ("0", "1", "2"):
+    for c in ("0", "1", "2", "3"):
         monkeypatch.setenv("SGW_AUTOTUNE_FORCE", c)
         out.append(op.apply(x, sig, group=grp))
     monkeypatch.delenv("SGW_AUTOTUNE_FORCE")
@@ -173,7 +173,7 @@ def test_autotuned_shapes_agree_bitwise(monkeypatch):
     for o in out[1:] + [tuned]:
         assert np.array_equal(o, out[0])
     L = _lib.lib()
-    assert L.sgw_autotune_choice(b"plane48/M8/b8/n32") in (0, 1, 2)
+    assert L.sgw_autotune_choice(b"plane48/M8/b8/n32") in (0, 1, 2, 3)
     assert L.sgw_autotune_choice(b"assemble/n32/g2/p36") in (0, 1, 2)
     ref = orc.PwOp(w, 0).apply(x[0, :w.npw[0]], sig[0])
     assert np.linalg.norm(tuned[0, :w.npw[0]] - ref) <= 1e-12 * np.linalg.norm(ref)
