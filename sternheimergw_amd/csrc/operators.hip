@@ -1673,8 +1673,8 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                                         std::to_string((int)decltype(mb)::value) + "/n" +
                                         std::to_string(size_class(nvec));
                 // shapes (planes, wavefronts): (3, 2) first where it fits (the
-                // fastest measured at 512 and 1024 vectors: 16 % less plane
-                // time than (1, 1)), then (1, 1), (1, 2), (2, 2)
+                // streamed 48^3 solve: 365.8k solves/s against 359.2k with
+                // (1, 1)), then (1, 1), (1, 2), (2, 2)
                 autotune(key, ncand, st, [&](int c) {
                   const int k = fit3 ? c : c + 1;
                   if (k == 0) launch_split(std::integral_constant<int, 3 * 8 + 2>{}, mb, no_rl);
