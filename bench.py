@@ -844,7 +844,7 @@ def main():
     happly_kernels = ("pw_becp_zinv", "pw_reduce_plane", "pw_zfwd", "pw_assemble")
     if os.environ.get("SGW_PW_FUSE") == "0":
         happly_kernels = ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble")
-    solver_kernels = ("bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed", "dotu")
+    solver_kernels = ("bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed")
     breakdown = None
     if args.breakdown:
         # per-kernel split from one extra, untimed step (events around every launch)

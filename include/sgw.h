@@ -527,7 +527,8 @@ int sgw_sigma_synthetic_device(int32_t n, int64_t ld, int32_t i0, int32_t ni, in
  * sgw_kernel_timing(level) sets the level and resets the totals: 0 off,
  * 1 one event pair per operator application ("happly"), 2 also one pair per
  * kernel launch ("pw_zinv", "pw_plane", "pw_zfwd", "pw_becp", "pw_assemble",
- * "bicg_v1", "bicg_seed", "shift_cycle", "mr_mgs", "mr_seed", "dotu").
+ * "bicg_v1", "bicg_seed" (the two seed halves of a BiCG step, each with its
+ * dot product), "shift_cycle", "mr_mgs", "mr_seed").
  * Query returns the summed duration (ms) and launch count of one name. */
 int sgw_kernel_timing(int enable);
 int sgw_kernel_timing_query(const char *name, double *total_ms, int64_t *launches);
