@@ -135,8 +135,9 @@ class PwOperator : public Operator {
   // last column holding plane waves; ncs_ columns per plane) and the full
   // grid (nslot_ columns, the layout of every other kernel)
   int ncs_ = 0;
-  DevBuf<int32_t> pk_slot_, pk_row_start_, pk_row_lo_, pk_row_cnt_;
-  DevBuf<int32_t> gr_slot_, gr_row_start_, gr_row_lo_, gr_row_cnt_;
+  DevBuf<int4> pk_info_, gr_info_;
+  DevBuf<int32_t> pk_row_start_, pk_row_lo_, pk_row_cnt_;
+  DevBuf<int32_t> gr_row_start_, gr_row_lo_, gr_row_cnt_;
 };
 
 }  // namespace sgw

@@ -78,11 +78,12 @@ struct PwDev {
   const int32_t *m3_of;        // [ngroup][ld] (reduced mod n3)
   const int32_t *col_lo;       // [ngroup][nslot] smallest m3 of the column
   // column layout of the split z / plane transforms' T and S: ncs columns per
-  // plane; packed column j of group g is grid slot col_slot[g][j] (-1: none);
+  // plane; packed column j of group g is grid slot col_info[g][j].x (-1:
+  // none), with that column's count, start and lowest m3 in .y, .z, .w;
   // row a of the grid holds columns m2 = row_lo .. row_lo + row_cnt - 1 at
   // j = row_start[g][a] + m2 - row_lo (row_start[g][nc1]: the group's count)
   int ncs;
-  const int32_t *col_slot;     // [ngroup][ncs]
+  const int4 *col_info;        // [ngroup][ncs]: (slot, col_cnt, col_start, col_lo)
   const int32_t *row_start;    // [ngroup][nc1 + 1]
   const int32_t *row_lo;       // [ngroup][nc1]
   const int32_t *row_cnt;      // [ngroup][nc1]
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(64) void k_pw_zfwd(PwDev d, const C *__restrict__ S
 // z transforms with each column split over a lane pair (splitfft.h): lane
 // 2j + odd of the pair of column j holds N/2 of its values, so these need
 // about half the registers of k_pw_zinv / k_pw_zfwd.  Columns are counted in
-// the layout of d.col_slot (T[v][z][j], d.ncs columns per plane).  Both lanes
+// the layout of d.col_info (T[v][z][j], d.ncs columns per plane).  Both lanes
 // of a pair take the same early exits.  MB >= 0: the sphere's z half-width
 // M3 as a compile-time constant, so values outside |m3| <= MB are literal
 // zeros (inverse) or never stored (forward); -1: no such bound.
@@ -330,9 +331,9 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
   const int ncs = d.ncs;
   if (jc >= ncs) return;
   const int g = group ? group[v] : 0;
-  const int c = d.col_slot[(int64_t)g * ncs + jc];
-  if (c < 0) return;  // past this group's columns
-  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  const int4 ci = d.col_info[(int64_t)g * ncs + jc];  // one load for the column's tables
+  if (ci.x < 0) return;  // past this group's columns
+  const int cnt = ci.y;
   C *Tv = T + (int64_t)v * N * ncs * kSlotScale<C> + jc;
   const int zo = odd ? Q : 0;  // this lane's z: zo + j and H + zo + j
   if (cnt == 0) {
@@ -343,7 +344,7 @@ __device__ inline void zinv_split_body(const PwDev &d, const double2 *__restrict
     }
     return;
   }
-  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int st = ci.z, lo = ci.w;
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
   const double2 *x = in + v * sin;
   C b[H];
@@ -376,11 +377,11 @@ __global__ __launch_bounds__(64) void k_pw_zfwd_split(PwDev d, const C *__restri
   const int ncs = d.ncs;
   if (jc >= ncs) return;
   const int g = group ? group[v] : 0;
-  const int c = d.col_slot[(int64_t)g * ncs + jc];
-  if (c < 0) return;  // past this group's columns
-  const int cnt = d.col_cnt[(int64_t)g * d.nslot + c];
+  const int4 ci = d.col_info[(int64_t)g * ncs + jc];
+  if (ci.x < 0) return;  // past this group's columns
+  const int cnt = ci.y;
   if (cnt == 0) return;
-  const int st = d.col_start[(int64_t)g * d.nslot + c], lo = d.col_lo[(int64_t)g * d.nslot + c];
+  const int st = ci.z, lo = ci.w;
   const int32_t *sidx = d.sorted_idx + (int64_t)g * d.ld + st;
   const C *Sv = S + (int64_t)v * N * ncs * kSlotScale<C> + jc;
   const int zo = odd ? Q : 0;
@@ -1431,24 +1432,31 @@ PwOperator::PwOperator(const PwDesc &desc) {
       ncs = std::max(ncs, j);
     }
     ncs_ = ncs;
-    std::vector<int32_t> cs((size_t)ngroup_ * ncs_, -1);
+    // (slot, count, start, lowest m3) of every packed column, (-1, 0, 0, 0)
+    // past a group's columns
+    auto info = [&](int g, int c) {
+      const size_t i = (size_t)g * nslot_ + c;
+      return make_int4(c, col_cnt[i], col_start[i], col_lo[i]);
+    };
+    std::vector<int4> cs((size_t)ngroup_ * ncs_, make_int4(-1, 0, 0, 0));
     for (int g = 0; g < ngroup_; ++g)
       for (int a = 0; a < nc1_; ++a)
         for (int k = 0; k < rc[(size_t)g * nc1_ + a]; ++k)
           cs[(size_t)g * ncs_ + rs[(size_t)g * nr + a] + k] =
-              a * nc2_ + rl[(size_t)g * nc1_ + a] + M2_ + k;
-    pk_slot_.alloc(cs.size()); pk_slot_.upload(cs.data(), cs.size());
+              info(g, a * nc2_ + rl[(size_t)g * nc1_ + a] + M2_ + k);
+    pk_info_.alloc(cs.size()); pk_info_.upload(cs.data(), cs.size());
     pk_row_start_.alloc(rs.size()); pk_row_start_.upload(rs.data(), rs.size());
     pk_row_lo_.alloc(rl.size()); pk_row_lo_.upload(rl.data(), rl.size());
     pk_row_cnt_.alloc(rc.size()); pk_row_cnt_.upload(rc.data(), rc.size());
     // the full grid in the same terms
-    std::vector<int32_t> gs((size_t)ngroup_ * nslot_), grs((size_t)ngroup_ * nr),
-        grl((size_t)ngroup_ * nc1_, -M2_), grc((size_t)ngroup_ * nc1_, nc2_);
+    std::vector<int4> gs((size_t)ngroup_ * nslot_);
+    std::vector<int32_t> grs((size_t)ngroup_ * nr), grl((size_t)ngroup_ * nc1_, -M2_),
+        grc((size_t)ngroup_ * nc1_, nc2_);
     for (int g = 0; g < ngroup_; ++g) {
-      for (int c = 0; c < nslot_; ++c) gs[(size_t)g * nslot_ + c] = c;
+      for (int c = 0; c < nslot_; ++c) gs[(size_t)g * nslot_ + c] = info(g, c);
       for (int a = 0; a <= nc1_; ++a) grs[(size_t)g * nr + a] = a * nc2_;
     }
-    gr_slot_.alloc(gs.size()); gr_slot_.upload(gs.data(), gs.size());
+    gr_info_.alloc(gs.size()); gr_info_.upload(gs.data(), gs.size());
     gr_row_start_.alloc(grs.size()); gr_row_start_.upload(grs.data(), grs.size());
     gr_row_lo_.alloc(grl.size()); gr_row_lo_.upload(grl.data(), grl.size());
     gr_row_cnt_.alloc(grc.size()); gr_row_cnt_.upload(grc.data(), grc.size());
@@ -1524,7 +1532,7 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
-  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.ncs = nslot_; d.col_info = gr_info_.p;
   d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
@@ -1584,7 +1592,7 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
       // vectors launches no blocks with dead planes among live ones
       int32_t *vl = (packed && d_active && wk.vlist) ? wk.vlist : nullptr;
       if (packed) {
-        d.ncs = ncs_; d.col_slot = pk_slot_.p;
+        d.ncs = ncs_; d.col_info = pk_info_.p;
         d.row_start = pk_row_start_.p; d.row_lo = pk_row_lo_.p; d.row_cnt = pk_row_cnt_.p;
       }
       const int64_t gxz = zsplit ? (d.ncs + 31) / 32 : (int64_t)gcol.x;
@@ -1822,7 +1830,7 @@ void PwOperator::drho_accumulate(hipStream_t st, int group, int nbnd, int nshift
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
-  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.ncs = nslot_; d.col_info = gr_info_.p;
   d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
@@ -1874,7 +1882,7 @@ void PwOperator::grad_accumulate(hipStream_t st, int np, const double2 *x, const
   d.col_start = col_start_.p; d.col_cnt = col_cnt_.p;
   d.sorted_idx = sidx_.p; d.sorted_m3 = sm3_.p; d.slot_of = slot_of_.p; d.m3_of = m3_of_.p;
   d.col_lo = col_lo_.p;
-  d.ncs = nslot_; d.col_slot = gr_slot_.p;
+  d.ncs = nslot_; d.col_info = gr_info_.p;
   d.row_start = gr_row_start_.p; d.row_lo = gr_row_lo_.p; d.row_cnt = gr_row_cnt_.p;
   d.kin = kin_.p; d.proj = proj_.p; d.dcoef = dcoef_.p;
   d.tw1 = tw_.p; d.tw2 = tw_.p + n1_; d.tw3 = tw_.p + n1_ + n2_;
