@@ -29,6 +29,9 @@ namespace sgw {
 namespace {
 
 constexpr int kBlock = 256;
+// slots from which a BiCG seed half step runs as one launch (one block per
+// system) rather than two (dot product, then the update over a grid of tiles)
+constexpr int kFuseHalves = 256;
 constexpr int kShiftGroups = 4;  // default shifted-system groups per k_shift_cycle block
 
 // ---------------------------------------------------------------------------
@@ -133,6 +136,23 @@ __device__ inline void bicg_s1_body(int s, int begin, int commit, Scal sc,
   sc.rho_old[s] = rho;
 }
 
+// rho = (r_j, r0~) by block s, then the scalar step above by its thread 0
+__global__ __launch_bounds__(kBlock) void k_dotu_s1(const int32_t *__restrict__ active,
+                                                    const double2 *__restrict__ a, int64_t sa,
+                                                    const double2 *__restrict__ b, int64_t n,
+                                                    int begin, int commit, Scal sc,
+                                                    int32_t *__restrict__ fresh,
+                                                    int32_t *__restrict__ count) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 acc = block_dotu(a + s * sa, b + s * n, n, lds);
+  if (threadIdx.x == 0) {
+    sc.dot[s] = acc;
+    bicg_s1_body(s, begin, commit, sc, fresh, count, acc);
+  }
+}
+
 // bicgstab.f90:614-670  alpha = rho / (u_{j+1}, r0~) and the shifted scalars
 // of shifted system k; `dot` = (u_{j+1}, r0~).  The per-step factors go to slot
 // (s*nsh + k)*l + j: the shifted vectors are updated once per iteration, for
@@ -163,6 +183,20 @@ __device__ inline void bicg_s2_body(int s, int k, int nsh, int l, int j, Scal sc
   sc.ia[pj] = cdiv(one, alpha_s);
 }
 
+// (u_{j+1}, r0~) by block s, then the shifted scalars of system s, one thread
+// per shifted system (the k = 0 thread also stores the seed alpha)
+__global__ __launch_bounds__(kBlock) void k_dotu_s2(const int32_t *__restrict__ active,
+                                                    const double2 *__restrict__ a, int64_t sa,
+                                                    const double2 *__restrict__ b, int64_t n,
+                                                    int nsh, int l, int j, Scal sc) {
+  __shared__ double2 lds[kBlock / 64];
+  const int s = blockIdx.x;
+  if (!active[s]) return;
+  const double2 acc = block_dotu(a + s * sa, b + s * n, n, lds);
+  if (threadIdx.x == 0) sc.dot[s] = acc;
+  for (int k = threadIdx.x; k < max(nsh, 1); k += kBlock) bicg_s2_body(s, k, nsh, l, j, sc, acc);
+}
+
 // ---------------------------------------------------------------------------
 // element-wise vector kernels; grid = (ceil(ld/kBlock), S)
 // ---------------------------------------------------------------------------
@@ -183,6 +217,47 @@ struct Vecs {
   __device__ double2 *su(int s, int k) const { return SU + ((int64_t)s * nsh + k) * ld; }
   __device__ double2 *sx(int s, int k) const { return SX + ((int64_t)s * nsh + k) * ld; }
 };
+
+// L7-L8: u_i = r_i - beta u_i, i = 0..j
+__global__ __launch_bounds__(kBlock) void k_bicg_v1(int j, const int32_t *__restrict__ active,
+                                                    Vecs v, Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  const double2 mb = cneg(sc.beta[s]);
+  for (int i = 0; i <= j; ++i) {
+    double2 *pu = v.u(s, i) + e;
+    double2 t = cmul(mb, *pu);
+    *pu = cadd(t, v.r(s, i)[e]);
+  }
+}
+
+// Seed half of step j after the matrix-vector product: L21-L22 (seed
+// residuals r_i -= alpha u_{i+1}) and L25 (x += alpha u_0).  The residuals
+// r_0..r_j before the update are kept as snapshots for the shifted systems
+// (their L15/L19 use them); after the last step, r_{l-1} after its update
+// (used by L26 of that step) is kept as well.
+__global__ __launch_bounds__(kBlock) void k_bicg_seed(int j, const int32_t *__restrict__ active,
+                                                      Vecs v, Scal sc) {
+  const int s = blockIdx.y;
+  if (!active[s]) return;
+  const int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (e >= v.ld) return;
+  const double2 alpha = sc.alpha_new[s];
+  const double2 malpha = cneg(alpha);
+  for (int i = 0; i <= j; ++i) {
+    double2 t = v.r(s, i)[e];
+    v.rs(s, snap_index(j, i))[e] = t;
+    cmac(t, malpha, v.u(s, i + 1)[e]);
+    v.r(s, i)[e] = t;
+    if (j == v.l - 1 && i == j) v.rs(s, snap_final(v.l))[e] = t;
+  }
+  double2 *px = v.x(s) + e;
+  double2 t = *px;
+  cmac(t, alpha, v.u(s, 0)[e]);
+  *px = t;
+}
 
 // The seed halves of BiCG step j around its H-apply, one block per system
 // (bicgstab.f90:594-608, and 613-616 with 672-686; the shifted scalars
@@ -1023,17 +1098,37 @@ struct BatchBicgstab::Impl {
     const int S = ns;
     const dim3 gv((unsigned)((ld + kBlock - 1) / kBlock), S);
     const int nmu = l * (l + 1) / 2, ntau = (l + 1) * (l + 2) / 2;
+    const int64_t sstride = (int64_t)(l + 1) * ld;
     // ---- bicg_part (Algorithm 3)
     for (int j = 0; j < l; ++j) {
+      // many slots: each seed half in one launch, one block per system; few
+      // (the lock-step tail): the dot product and the update over a grid of
+      // tiles as two launches, which spreads a system over more CUs.  Both
+      // give the same bits.
+      const bool fused = S >= kFuseHalves;
       {
         KScope ks_("bicg_v1", st, 2);
-        k_bicg_half1<<<S, kBlock, 0, st>>>(j, active.p, vv, j == 0 ? 1 : 0, 1, sc,
-                                           j == 0 ? fresh.p : nullptr, j == 0 ? count.p : nullptr);
+        if (fused) {
+          k_bicg_half1<<<S, kBlock, 0, st>>>(j, active.p, vv, j == 0 ? 1 : 0, 1, sc,
+                                             j == 0 ? fresh.p : nullptr,
+                                             j == 0 ? count.p : nullptr);
+        } else {
+          k_dotu_s1<<<S, kBlock, 0, st>>>(active.p, R.p + j * ld, sstride, RT.p, ld,
+                                          j == 0 ? 1 : 0, 1, sc, j == 0 ? fresh.p : nullptr,
+                                          j == 0 ? count.p : nullptr);
+          k_bicg_v1<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
+        }
       }
       op_apply(op, st, U.p + j * ld, U.p + (j + 1) * ld);       // L10
       {
         KScope ks_("bicg_seed", st, 2);
-        k_bicg_half2<<<S, kBlock, 0, st>>>(j, active.p, vv, nsh, l, sc);
+        if (fused) {
+          k_bicg_half2<<<S, kBlock, 0, st>>>(j, active.p, vv, nsh, l, sc);
+        } else {
+          k_dotu_s2<<<S, kBlock, 0, st>>>(active.p, U.p + (j + 1) * ld, sstride, RT.p, ld, nsh,
+                                          l, j, sc);
+          k_bicg_seed<<<gv, kBlock, 0, st>>>(j, active.p, vv, sc);
+        }
       }
       op_apply(op, st, R.p + j * ld, R.p + (j + 1) * ld);       // L24
     }
