@@ -1272,11 +1272,10 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
                                         reinterpret_cast<C *>(smem_d), relax, want, vlist);
 }
 
-// planes per block and wavefronts per block of the split plane kernel.
-// Measured per H-apply on the 48^3 batch: one plane on one wavefront (the X
-// stage in 1.5 rounds of 32 lane pairs, 13 KB of LDS) is 4 % faster than two
-// planes on two wavefronts, 12 % faster than two on three and 15 % faster
-// than four on three (every lane pair busy, but 52 KB of LDS per block).
+// planes per block and wavefronts per block of the split plane kernel where
+// no other shape is tried: one plane on one wavefront (the X stage in 1.5
+// rounds of 32 lane pairs, 13 KB of LDS).  The 48 box's autotuned shapes
+// come first at the launch (three planes on two wavefronts, then this).
 template <int N> struct SplitShape {
   static constexpr int PL = 1;
   static constexpr int W = 1;
@@ -1654,7 +1653,8 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                     ngroup_, nred_x, wk.part, wk.becp, T32, S32, nvec, rl, 1, vl);
           };
           const bool band = band_planes_ && M1_ == M2_;
-          static_assert(PL == 1 && W == 1, "the packed-layout test assumes one plane per wave");
+          static_assert(PL == 1 && W == 1,
+                        "split_plane_ok is the default shape's fit test for one plane per wave");
           if (packed) {  // split_planes_, nc1_ <= 32 and lds_split <= 64 KB
             SGW_REQUIRE(lds_split <= 64 * 1024 && PL * nc1_ <= 32 * W, SGW_E_STATE,
                         "pw operator: split plane kernel does not fit");
