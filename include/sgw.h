@@ -533,14 +533,14 @@ int sgw_sigma_synthetic_device(int32_t n, int64_t ld, int32_t i0, int32_t ni, in
 int sgw_kernel_timing(int enable);
 int sgw_kernel_timing_query(const char *name, double *total_ms, int64_t *launches);
 
-/* Launch-shape autotuning.  No counterpart in the reference.  The plane
- * transforms on the 48 box (three planes on two wavefronts, one plane per
- * wavefront, one plane on two, two planes on two) and the final assembly of
+/* Launch-shape autotuning.  No counterpart in the reference.  The split plane
+ * transforms of cubic boxes up to 48 (three planes on two wavefronts, one
+ * plane per wavefront, one plane on two, two planes on two) and the final assembly of
  * an application (16, 32 or 64 plane waves per block) have several launch
  * shapes with the same results; the first application of each size class (batch size
  * rounded up to a power of two) times every shape once and keeps the fastest.
  * SGW_AUTOTUNE=0 in the environment keeps the first shape.  Returns the
- * candidate index chosen for `key` ("plane48/M<m>/b<band>/n<class>",
+ * candidate index chosen for `key` ("plane<box>/M<m>/b<band>/n<class>",
  * "assemble/n<class>/g<groups>/p<projectors>"), -1 if none yet. */
 int sgw_autotune_choice(const char *key);
 

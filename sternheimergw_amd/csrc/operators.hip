@@ -1273,9 +1273,10 @@ __global__ __launch_bounds__(64 * W) void k_pw_reduce_split(
 }
 
 // planes per block and wavefronts per block of the split plane kernel where
-// no other shape is tried: one plane on one wavefront (the X stage in 1.5
-// rounds of 32 lane pairs, 13 KB of LDS).  The 48 box's autotuned shapes
-// come first at the launch (three planes on two wavefronts, then this).
+// no other shape is tried (mixed-precision batches): one plane on one
+// wavefront (on the 48 box the X stage in 1.5 rounds of 32 lane pairs, 13 KB
+// of LDS).  FP64 batches take the autotuned shapes at the launch (three
+// planes on two wavefronts first, then this).
 template <int N> struct SplitShape {
   static constexpr int PL = 1;
   static constexpr int W = 1;
@@ -1659,11 +1660,11 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
             SGW_REQUIRE(lds_split <= 64 * 1024 && PL * nc1_ <= 32 * W, SGW_E_STATE,
                         "pw operator: split plane kernel does not fit");
             // compile-time sphere half-widths for the usual ones of this box;
-            // on the 48 box the autotuner picks among three planes on two
-            // wavefronts (Y stage on 51 of 64 lane pairs, X stage in 2.25
+            // the autotuner picks among three planes on two wavefronts (on
+            // the 48 box: Y stage on 51 of 64 lane pairs, X stage in 2.25
             // rounds), one plane on one, one plane on two (the X stage in one
-            // round) and two planes on two for this size class of the batch
-            // (mode 2: one plane on one)
+            // round) and two planes on two for this box and size class of the
+            // batch (mode 2: one plane on one)
             with_band<NB>(band, M1_, [&](auto mb) {
               using I11 = std::integral_constant<int, PL * 8 + W>;
               constexpr std::false_type no_rl{};
@@ -1672,12 +1673,13 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                 launch_split(I11{}, mb, std::true_type{});
                 return;
               }
-              if constexpr (NB == 48) {
+              {
                 const size_t row = sizeof(double2) * (size_t)nc1_ * (NB + 1);
                 const bool fit2 = nc1_ <= 32 && 2 * row <= 64 * 1024;
                 const bool fit3 = fit2 && 3 * nc1_ <= 64 && 3 * row <= 64 * 1024;
                 const int ncand = fit2 ? (fit3 ? 4 : 3) : 1;
-                const std::string key = "plane48/M" + std::to_string(M1_) + "/b" +
+                const std::string key = "plane" + std::to_string(NB) + "/M" +
+                                        std::to_string(M1_) + "/b" +
                                         std::to_string((int)decltype(mb)::value) + "/n" +
                                         std::to_string(size_class(nvec));
                 // shapes (planes, wavefronts): (3, 2) first where it fits (the
@@ -1691,8 +1693,6 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                     launch_split(std::integral_constant<int, 1 * 8 + 2>{}, mb, no_rl);
                   else launch_split(std::integral_constant<int, 2 * 8 + 2>{}, mb, no_rl);
                 });
-              } else {
-                launch_split(I11{}, mb, no_rl);
               }
             });
             if (run32) with_band<NB>(band, M1_, [&](auto mb) { launch_split32(mb); });
