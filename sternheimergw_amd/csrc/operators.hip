@@ -929,7 +929,7 @@ __global__ __launch_bounds__(64) void k_pw_plane_grad(PwDev d, const double2 *__
 // ---------------------------------------------------------------------------
 constexpr int kChunk = 64;
 constexpr int kVT = 16;
-constexpr int kAsmChunk = 32, kAsmThreads = 256;  // k_pw_assemble2 block shape
+constexpr int kAsmChunk = 64, kAsmThreads = 256;  // k_pw_assemble2 default block shape
 constexpr int kPT = 16;  // projectors staged in LDS at a time
 
 // does any vector of the tile still run?  (uniform across the block)
@@ -1781,11 +1781,12 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
   {
     const size_t lds_asm = sizeof(double2) * ((size_t)nproj_ * kVT + n3_);
     KScope ks("pw_assemble", st, 2);
-    // 32 plane waves x 8 vector groups per 256-lane block: measured per
-    // H-apply on the 48^3 batch, 2 % faster than 64 or 16 plane waves per
-    // 256 lanes and 2-5 % faster than 8 or 16 per 64 lanes; the autotuner
-    // tries 16 and 64 as well for each size class of the batch (not when the
-    // output overwrites the input, which a second launch would read)
+    // 64 plane waves x 4 vector groups per 256-lane block by default: on the
+    // 48^3 batch 21.9 us per 512 vectors against 26.4 with 32 plane waves x 8
+    // groups and 33.2 with 16 x 16, and the one the autotuner keeps there;
+    // the autotuner tries 32 and 16 as well for each size class of the batch
+    // (not when the output overwrites the input, which a second launch would
+    // read)
     constexpr int NT = kAsmThreads;
     auto launch_asm = [&](auto chunk) {
       constexpr int CH = decltype(chunk)::value;
@@ -1800,8 +1801,8 @@ void PwOperator::apply_work(hipStream_t st, int nvec, const double2 *in, int64_t
                             std::to_string(ngroup_) + "/p" + std::to_string(nproj_);
     autotune(key, disjoint ? 3 : 1, st, [&](int c) {
       if (c == 0) launch_asm(std::integral_constant<int, kAsmChunk>{});
-      else if (c == 1) launch_asm(std::integral_constant<int, 16>{});
-      else launch_asm(std::integral_constant<int, 64>{});
+      else if (c == 1) launch_asm(std::integral_constant<int, 32>{});
+      else launch_asm(std::integral_constant<int, 16>{});
     });
   }
   SGW_HIP(hipGetLastError());
