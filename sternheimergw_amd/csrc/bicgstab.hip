@@ -30,8 +30,9 @@ namespace {
 
 constexpr int kBlock = 256;
 // slots from which a BiCG seed half step runs as one launch (one block per
-// system) rather than two (dot product, then the update over a grid of tiles)
-constexpr int kFuseHalves = 256;
+// system) rather than two (dot product, then the update over a grid of
+// tiles): the one-launch form is faster at 1024 slots, slower at 256
+constexpr int kFuseHalves = 512;
 constexpr int kShiftGroups = 4;  // default shifted-system groups per k_shift_cycle block
 
 // ---------------------------------------------------------------------------
